@@ -1,0 +1,199 @@
+"""Benchmark: photon steps/s on one MI355X for the 64-geometry scale workload.
+
+Workload (one "step"): a point emitter with a Cherenkov cone about +z
+(λ ∝ 1/λ² on 300-600 nm, β = 1) in the seawater-like medium (n 1.36→1.34,
+λ_abs 10/30/10 m at 300/450/600 nm, λ_sca = 50 m·(λ/450 nm)^4, g = 0.92),
+1e9 photons towards each of 64 spheres (r = 0.21 m): 16 distances
+geomspace(1, 100, 16) m × 4 directions at 0°, 45°, 90°, 135° from the emitter
+axis, max_steps = 10, Philox streams keyed by (seed, geometry).  All 64
+geometries run as one persistent batched launch; hits are compacted per
+geometry in device memory.
+
+Prints one JSON line (see the fields below).  --gpus N with N > 1 exits: this
+build measures one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "photonprop.jl_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+FLOPS_PER_STEP = 81          # canonical FP32 flops per scattering-loop iteration
+PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 peak
+HIT_BYTES = 64
+
+
+def workload_requests(pp, photons_per_geometry, seed=0):
+    dists = np.geomspace(1.0, 100.0, 16)
+    angles = (0.0, 45.0, 90.0, 135.0)
+    reqs = pp.make_requests(len(dists) * len(angles))
+    i = 0
+    for d in dists:
+        for a in angles:
+            ar = math.radians(a)
+            reqs["pos"][i] = (0.0, 0.0, 0.0)
+            reqs["axis"][i] = (0.0, 0.0, 1.0)
+            reqs["beta"][i] = 1.0
+            reqs["n_photons"][i] = photons_per_geometry
+            reqs["profile"][i] = pp._abi.PP_EMIT_CHERENKOV_CONE
+            reqs["center"][i] = (d * math.sin(ar), 0.0, d * math.cos(ar))
+            reqs["radius"][i] = 0.21
+            reqs["seed"][i] = seed
+            reqs["stream"][i] = i
+            reqs["max_steps"][i] = 10
+            i += 1
+    return reqs
+
+
+def cpu_baseline(pp, photons_per_geometry_sample):
+    """The oracle (FP32, OpenMP over all host cores this process may use) on
+    the same 64 geometries with a bounded photon sample per geometry."""
+    import oracle
+    from oracle.compare import oracle_for
+
+    tables = pp.medium_tables(pp.SeawaterConfigMedium())
+    reqs = workload_requests(pp, photons_per_geometry_sample)
+    threads = len(os.sched_getaffinity(0))
+    oracle.lib()
+    steps = hits = 0
+    t0 = time.perf_counter()
+    for r in reqs:
+        src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0, int(r["n_photons"]),
+                              pp.InverseSquareSpectrum(300.0, 600.0))
+        tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
+        _, s = oracle_for(tables, src, tgt, seed=int(r["seed"]), stream=int(r["stream"]),
+                          n_threads=threads)
+        steps += s["n_steps"]
+        hits += s["n_hits"]
+    dt = time.perf_counter() - t0
+    return {
+        "value": steps / dt,
+        "unit": "photon steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "64 geometries x %d photons (%d steps, %d hits) in %.1f s; detected photons/s %.4g"
+                  % (photons_per_geometry_sample, steps, hits, dt, hits / dt),
+        "detected_photons_per_s": hits / dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons", type=float, default=1e9, help="photons per geometry")
+    ap.add_argument("--cpu-sample", type=float, default=1e6,
+                    help="photons per geometry for the CPU baseline (0 skips it)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; --gpus %d is not supported (the path shards by photon-id "
+              "range across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import photonprop as pp
+
+    n_per = int(args.photons)
+    reqs = workload_requests(pp, n_per)
+    spectrum = pp.InverseSquareSpectrum(300.0, 600.0)
+    ctx = pp.Context(pp.SeawaterConfigMedium(), device=0)
+    # ~0.26 hits per 1e3 photons over the 64 geometries; keep 2x headroom
+    cap = int(max(1 << 20, 6.0e-4 * n_per * len(reqs)))
+    d_hits = torch.empty((cap, HIT_BYTES), dtype=torch.uint8, device="cuda:0")
+    d_off = torch.zeros(len(reqs) + 1, dtype=torch.int64, device="cuda:0")
+    d_sum = torch.zeros((len(reqs), 64), dtype=torch.uint8, device="cuda:0")
+    spec, keep = pp.propagation.spectrum_struct(spectrum)
+    lib = ctx.lib
+    creqs = reqs.ctypes.data_as(pp._abi.ctypes.POINTER(pp._abi.pp_request))
+
+    def step():
+        pp._abi.check(lib.pp_propagate_batch_device(
+            ctx.handle, creqs, len(reqs), pp._abi.ctypes.byref(spec),
+            pp._abi.ctypes.c_void_p(d_hits.data_ptr()), cap,
+            pp._abi.ctypes.c_void_p(d_off.data_ptr()), pp._abi.ctypes.c_void_p(d_sum.data_ptr())), lib)
+
+    for i in range(args.warmup):
+        step()
+        pp._abi.check(lib.pp_synchronize(ctx.handle), lib)
+        print("warmup %d done" % i, file=sys.stderr, flush=True)
+
+    kernel_ms = []
+    torch.cuda.synchronize()
+    pp._abi.check(lib.pp_synchronize(ctx.handle), lib)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        kernel_ms.append(ctx.last_kernel_ms())  # waits for this step's kernel
+        print("step %d: kernel %.1f ms" % (i, kernel_ms[-1]), file=sys.stderr, flush=True)
+    pp._abi.check(lib.pp_synchronize(ctx.handle), lib)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    sums = np.frombuffer(d_sum.cpu().numpy().tobytes(), dtype=pp._abi.SUMMARY_DTYPE)
+    steps_per_launch = int(sums["n_steps"].sum())
+    hits_per_launch = int(sums["n_hits"].sum())
+    stored = int(sums["n_stored"].sum())
+    assert stored == hits_per_launch, "hit buffer overflow in the benchmark"
+    assert int(sums["n_sim"].sum()) == n_per * len(reqs)
+    ms_per_step = 1e3 * elapsed / args.steps
+    steps_per_s = steps_per_launch * args.steps / elapsed
+    avg_kernel_s = 1e-3 * sum(kernel_ms) / len(kernel_ms)
+    achieved = steps_per_launch * FLOPS_PER_STEP / avg_kernel_s / 1e12
+    name, n_cu = ctx.device_info()
+
+    out = {
+        "metric": "photon steps/s + detected photons/s at 1 GPU; p50/p99 request latency; surrogate err",
+        "value": steps_per_s,
+        "unit": "photon steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "64 geometries (16 distances geomspace(1,100,16) m x 0/45/90/135 deg), "
+                        "%.3g photons each, seawater-like medium, Cherenkov cone 300-600 nm, "
+                        "sphere r=0.21 m, max_steps 10, one batched persistent launch" % n_per,
+            "photons_per_step": n_per * len(reqs),
+            "parallelism": "single GPU",
+        },
+        "detected_photons_per_s": hits_per_launch * args.steps / elapsed,
+        "photons_per_s": n_per * len(reqs) * args.steps / elapsed,
+        "photon_steps_per_launch": steps_per_launch,
+        "hits_per_launch": hits_per_launch,
+        "kernel_ms_avg": 1e3 * avg_kernel_s,
+        "device": "%s, %d CUs" % (name, n_cu),
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "note": "FP32 vector ALU (not MFMA): %d flops per photon step x steps per launch / "
+                    "average propagate_kernel duration (HIP events on the library's stream)"
+                    % FLOPS_PER_STEP,
+        },
+    }
+    if args.cpu_sample > 0:
+        cb = cpu_baseline(pp, int(args.cpu_sample))
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = steps_per_s / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,241 @@
+/*
+ * photonprop.h — C ABI of the MI355X (gfx950) photon-propagation library.
+ *
+ * The library replaces the device side of PhotonProp.jl's propagation path:
+ *   - the kernel cuda_propagate_photons!      (src/photon_prop_cuda.jl:262-386)
+ *   - its host driver propagate_distance      (src/photon_prop_cuda.jl:588-644)
+ *   - the buffer helpers initialize_photon_arrays / process_output
+ *                                              (src/photon_prop_cuda.jl:390-422)
+ * Everything crossing this boundary is plain C: fixed-layout structs, raw
+ * pointers and sizes.  No C++ or torch types.
+ *
+ * Units: metres, nanometres, nanoseconds.
+ *
+ * Conventions
+ *   - Every function returns PP_OK (0) or a negative pp_status code.  The
+ *     message of the last failure on the calling thread is pp_last_error().
+ *   - A pp_ctx owns all device memory it uses (medium tables, hit staging,
+ *     per-tile partials).  The caller owns every host buffer it passes in.
+ *   - Calls on one pp_ctx are synchronous unless the name ends in _device, and
+ *     a pp_ctx must not be used from two host threads at once.  Use one
+ *     pp_ctx per host thread; each has its own HIP stream.
+ *   - Photon i of a request draws its random numbers from the Philox4x32-10
+ *     counter stream keyed by (seed) with counter (block, i_lo, i_hi, stream),
+ *     so a photon's history depends only on (seed, stream, i, inputs) and not
+ *     on launch geometry, batch composition or device.
+ */
+#ifndef PHOTONPROP_H
+#define PHOTONPROP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_ABI_VERSION 1
+
+typedef enum pp_status {
+    PP_OK = 0,
+    PP_ERR_INVALID_ARG = -1, /* bad argument; see pp_last_error()                   */
+    PP_ERR_OVERFLOW = -2,    /* hit buffer too small: counts and sums are exact,
+                                stored records are truncated to the capacity       */
+    PP_ERR_DEVICE = -3,      /* HIP runtime error                                   */
+    PP_ERR_OOM = -4,         /* device allocation failed                            */
+    PP_ERR_UNSUPPORTED = -5, /* unknown spectrum kind or emission profile; mirrors
+                                the reference's ArgumentError("Cannot initialize")
+                                (src/photon_prop_cuda.jl:67, :70)                   */
+    PP_ERR_BUSY = -6,        /* serving queue full / capacity exceeded (back-pressure) */
+    PP_ERR_NOT_FOUND = -7    /* unknown request ticket                              */
+} pp_status;
+
+/* Spectrum kinds (reference: src/spectrum.jl:19-103). */
+typedef enum pp_spectrum_kind {
+    PP_SPEC_MONO = 0,        /* Monochromatic (src/spectrum.jl:21-23)                 */
+    PP_SPEC_KNOTS = 1,       /* CherenkovSpectrum: inverse-CDF knots equidistant in u
+                                on [0,1] (src/spectrum.jl:90-103, utils.jl:37-60)   */
+    PP_SPEC_INV_LAMBDA2 = 2  /* dN/dλ ∝ 1/λ² on [lo,hi], closed-form inverse CDF    */
+} pp_spectrum_kind;
+
+/* Emission profiles (reference: AngularEmissionProfile, src/emission.jl:13). */
+typedef enum pp_emission_profile {
+    PP_EMIT_CHERENKOV_CONE = 0, /* cosθ = 1/(n(λ)·β) about the emitter axis, φ uniform */
+    PP_EMIT_ISOTROPIC = 1,      /* uniform on the sphere (cosθ uniform on [-1,1])       */
+    PP_EMIT_REF_ISOTROPIC = 2   /* the reference's :IsotropicEmission as it behaves:
+                                   θ ~ U(-1,1) radians, φ ~ U(0,2π), axis ignored
+                                   (src/photon_prop_cuda.jl:52-56)                     */
+} pp_emission_profile;
+
+/* Medium as tables on a uniform wavelength grid of n_points points spanning
+ * [lambda_min, lambda_max] nm.  Replaces WaterProperties + get_refractive_index /
+ * get_scattering_length / get_absorption_length (src/medium.jl:23-251), which
+ * the reference bakes into the kernel as a compile-time Val(medium)
+ * (src/photon_prop_cuda.jl:274, :609).  Values are linearly interpolated and
+ * clamped at the grid ends. */
+typedef struct pp_medium_tables {
+    float lambda_min;       /* nm */
+    float lambda_max;       /* nm, > lambda_min */
+    uint32_t n_points;      /* 2 .. PP_MAX_TABLE_POINTS */
+    uint32_t reserved;
+    const float* n_phase;   /* [n_points] phase refractive index            */
+    const float* abs_len;   /* [n_points] absorption length, m (> 0)        */
+    const float* sca_len;   /* [n_points] scattering length, m (> 0)        */
+    float g;                /* Henyey-Greenstein asymmetry, |g| < 1          */
+    float reserved2;
+} pp_medium_tables;
+
+#define PP_MAX_TABLE_POINTS 2048
+#define PP_MAX_KNOTS 1024
+
+/* Wavelength spectrum (replaces PhotonSource.spectrum, src/emission.jl:107). */
+typedef struct pp_spectrum {
+    uint32_t kind;          /* pp_spectrum_kind */
+    uint32_t n_knots;       /* PP_SPEC_KNOTS: 2 .. PP_MAX_KNOTS */
+    float lo, hi;           /* PP_SPEC_INV_LAMBDA2: range in nm */
+    float mono;             /* PP_SPEC_MONO: wavelength in nm */
+    float reserved;
+    const float* knots;     /* PP_SPEC_KNOTS: λ at u = k/(n_knots-1) */
+} pp_spectrum;
+
+/* Point emitter (replaces PhotonSource, src/emission.jl:102-109). */
+typedef struct pp_emitter {
+    float pos[3];           /* m  */
+    float t0;               /* ns */
+    float axis[3];          /* unit vector (normalised by the library) */
+    float beta;             /* particle speed / c for the Cherenkov cone */
+    uint64_t n_photons;     /* photons to simulate, exactly */
+    uint32_t profile;       /* pp_emission_profile */
+    uint32_t reserved;
+} pp_emitter;
+
+/* Spherical target (replaces DetectionSphere, src/detection.jl:12-15). */
+typedef struct pp_sphere {
+    float center[3];        /* m */
+    float radius;           /* m, > 0 */
+} pp_sphere;
+
+/* One detected photon, 64 bytes.  Replaces the reference's four SoA hit stacks
+ * (position, initial direction, wavelength, distance; src/photon_prop_cuda.jl:369-372)
+ * and adds what the reference computes on the host afterwards (weight, time). */
+typedef struct pp_hit {
+    float pos[3];           /* hit point on the sphere, m */
+    float time;             /* t0 + L·n(λ)/c, ns (phase index, src/photon_prop_cuda.jl:634-638) */
+    float dir[3];           /* direction at arrival */
+    float wavelength;       /* nm */
+    float init_dir[3];      /* direction at emission */
+    float weight;           /* absorption weight exp(-L/λ_abs(λ)) */
+    float dist;             /* path length L, m */
+    uint32_t request;       /* request index within the launch (0 for pp_propagate) */
+    uint64_t photon_id;     /* photon index within its request */
+} pp_hit;
+
+/* Per-request result summary.  Sums accumulate in FP64 in a fixed order that
+ * depends only on the request itself. */
+typedef struct pp_summary {
+    uint64_t n_sim;         /* photons simulated (== n_photons) */
+    uint64_t n_steps;       /* scattering-loop iterations executed */
+    uint64_t n_hits;        /* photons that reached the target (exact) */
+    uint64_t n_stored;      /* hit records written to the output */
+    double sum_w;           /* Σ weight */
+    double sum_wt;          /* Σ weight·time */
+    double sum_wt2;         /* Σ weight·time² */
+    int32_t status;         /* PP_OK or PP_ERR_OVERFLOW */
+    uint32_t reserved;
+} pp_summary;
+
+/* One propagation request: emitter + target + random stream. */
+typedef struct pp_request {
+    pp_emitter emitter;
+    pp_sphere target;
+    uint64_t seed;          /* Philox key */
+    uint32_t stream;        /* Philox counter word 3 */
+    uint32_t max_steps;     /* scattering-loop cap; reference uses 10 (src/photon_prop_cuda.jl:319) */
+} pp_request;
+
+typedef struct pp_ctx pp_ctx;
+
+/* ---- library / context ---------------------------------------------------- */
+
+/* Thread-local message of the last failed call. */
+const char* pp_last_error(void);
+
+/* PP_ABI_VERSION of the loaded library. */
+int pp_abi_version(void);
+
+/* Create a context on HIP device `device` and upload the medium tables once.
+ * Replaces the reference's compile-time Val(medium) and its module-load warm-up
+ * _init_workaround (src/photon_prop_cuda.jl:646-654). */
+int pp_create(pp_ctx** out, int device, const pp_medium_tables* medium);
+
+/* Replace the resident medium tables. */
+int pp_set_medium(pp_ctx* ctx, const pp_medium_tables* medium);
+
+int pp_destroy(pp_ctx* ctx);
+
+/* Name and compute-unit count of the context's device (name_cap bytes incl. NUL). */
+int pp_device_info(pp_ctx* ctx, char* name, size_t name_cap, int* n_cu);
+
+/* ---- plain propagation ---------------------------------------------------- */
+
+/* Propagate emitter->n_photons photons towards one sphere and return their hits.
+ * Replaces @cuda cuda_propagate_photons! + process_output + the host weight
+ * pass of propagate_distance (src/photon_prop_cuda.jl:262-386, :588-644).
+ * hits_out: host buffer of hits_cap records (may be NULL when hits_cap == 0).
+ * Returns PP_ERR_OVERFLOW (summary still exact) if more photons hit than fit. */
+int pp_propagate(pp_ctx* ctx, const pp_emitter* emitter, const pp_spectrum* spectrum,
+                 const pp_sphere* target, uint64_t seed, uint32_t stream, uint32_t max_steps,
+                 pp_hit* hits_out, uint64_t hits_cap, pp_summary* summary_out);
+
+/* ---- batched propagation (one persistent launch for many requests) --------- */
+
+/* Propagate n_req requests of uneven size in one launch.  The reference's only
+ * batching is the unfinished split_source / propagate_sources
+ * (src/photon_prop_cuda.jl:474-531).  Hits come back grouped by request:
+ * request r owns hits_out[offsets_out[r] .. offsets_out[r+1]).
+ * offsets_out: n_req+1 entries; summaries_out: n_req entries. */
+int pp_propagate_batch(pp_ctx* ctx, const pp_request* requests, uint32_t n_req,
+                       const pp_spectrum* spectrum, pp_hit* hits_out, uint64_t hits_cap,
+                       uint64_t* offsets_out, pp_summary* summaries_out);
+
+/* Device-resident form of pp_propagate_batch: enqueue on the context's stream and
+ * return.  d_hits (hits_cap records), d_offsets (n_req+1) and d_summaries (n_req)
+ * are device pointers; results are grouped by request as above and are valid
+ * after pp_synchronize(ctx).  Requests are copied before the call returns. */
+int pp_propagate_batch_device(pp_ctx* ctx, const pp_request* requests, uint32_t n_req,
+                              const pp_spectrum* spectrum, pp_hit* d_hits, uint64_t hits_cap,
+                              uint64_t* d_offsets, pp_summary* d_summaries);
+
+/* Wait for all work enqueued on the context's stream. */
+int pp_synchronize(pp_ctx* ctx);
+
+/* Duration in milliseconds of the last propagation kernel launched on this
+ * context, measured with HIP events on the context's stream. */
+int pp_last_kernel_ms(pp_ctx* ctx, float* ms);
+
+/* ---- device self-tests of the kernel's own helper functions ---------------- */
+/* The reference checks its device helpers through tiny test kernels
+ * (apply_rot, src/tests/test_photon_prob.jl:32-41).  These run the helpers the
+ * propagation kernel inlines, on the device, over host arrays. */
+
+/* out[3i..3i+2] = rotate_to_axis(old_dirs[3i..], new_dirs[3i..]) (src/photon_prop_cuda.jl:171-219). */
+int pp_selftest_rotate(pp_ctx* ctx, const float* old_dirs, const float* new_dirs, float* out,
+                       uint64_t n);
+
+/* out[i] = Henyey-Greenstein cosθ for g from Philox word i of (seed, stream 0)
+ * (src/photon_prop_cuda.jl:31-38). */
+int pp_selftest_hg(pp_ctx* ctx, float g, uint64_t seed, float* out, uint64_t n);
+
+/* words_out[4i..4i+3] = Philox4x32-10(counter = ctr[4i..4i+3], key = (seed_lo, seed_hi)). */
+int pp_selftest_philox(pp_ctx* ctx, const uint32_t* ctr, uint64_t seed, uint32_t* words_out,
+                       uint64_t n);
+
+/* Sums over n draws of HG cosθ computed on the device in FP64:
+ * sums_out = {Σcosθ, Σcos²θ}.  For the mean-cosine check at ≥1e7 draws. */
+int pp_selftest_hg_moments(pp_ctx* ctx, float g, uint64_t seed, uint64_t n, double* sums_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTONPROP_H */

@@ -1,0 +1,132 @@
+// pp_device.hpp — device helpers inlined by the gfx950 propagation kernel.
+//
+// Built with -ffp-contract=off: every fused multiply-add below is written out,
+// so the operation order is fixed and matches the CPU oracle wherever the two
+// use the same operations.  Per-step transcendentals use the hardware
+// instructions (v_log_f32, v_sin_f32/v_cos_f32 in revolutions, v_rcp_f32,
+// v_rsq_f32, v_sqrt_f32); per-photon and per-hit math uses IEEE division and
+// square root.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ppd {
+
+// Philox4x32 constants (Salmon et al., SC'11).
+constexpr uint32_t kM0 = 0xD2511F53u;
+constexpr uint32_t kM1 = 0xCD9E8D57u;
+constexpr uint32_t kW0 = 0x9E3779B9u;
+constexpr uint32_t kW1 = 0xBB67AE85u;
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+// Philox4x32-10.  The key (k0, k1) is wave-uniform, so its schedule lives in
+// scalar registers; each round costs two 32x32->64 multiplies and two 3-input
+// XORs per lane.
+__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)kM0 * c0;
+        const uint64_t p1 = (uint64_t)kM1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += kW0;
+        k1 += kW1;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// Uniform on (0, 1]: (top 24 bits + 1) * 2^-24, exact in FP32.
+__device__ __forceinline__ float u01(uint32_t w) {
+    return __builtin_fmaf((float)(w >> 8), 0x1p-24f, 0x1p-24f);
+}
+
+// Henyey-Greenstein cosθ by inverse CDF (src/photon_prop_cuda.jl:31-38):
+// cosθ = (1+g² - ((1-g²)/(1+g(2η-1)))²)/(2g), clamped; g == 0 is isotropic.
+struct HG {
+    float g, num, opg2, inv2g;
+    int iso;
+};
+
+__device__ __forceinline__ float hg_costheta(const HG& h, float eta) {
+    const float x = __builtin_fmaf(2.0f, eta, -1.0f);
+    if (h.iso) return x;
+    const float den = __builtin_fmaf(h.g, x, 1.0f);
+    const float q = h.num * __builtin_amdgcn_rcpf(den);
+    float c = h.inv2g * (h.opg2 - q * q);
+    c = fminf(fmaxf(c, -1.0f), 1.0f);
+    return c;
+}
+
+// rotate_to_axis (src/photon_prop_cuda.jl:171-219): apply to (nx,ny,nz) the
+// rotation taking e_z to (x,y,z), then renormalise.  The reference's form
+// divides by sinθ = sqrt(1-z²) twice; with q = (x·ny - y·nx)/(1+z) the same
+// rotation reads
+//     x' = nx·z + x·nz - y·q,  y' = ny·z + y·nz + x·q,  z' = nz·z - (x·nx + y·ny).
+// For z < 0, 1+z is taken as (x²+y²)/(1-z), which stays consistent with x, y
+// near the south pole.  At the exact pole (x = y = 0, z < 0) the result is the
+// reference's (nx, -ny, -nz).
+__device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float nx, float ny,
+                                               float nz, float& ox, float& oy, float& oz) {
+    const float s2 = __builtin_fmaf(y, y, x * x);
+    const bool up = z >= 0.0f;
+    const float w = up ? 1.0f + z : s2;
+    const float m = up ? 1.0f : 1.0f - z;
+    const float inv_h = m * __builtin_amdgcn_rcpf(w);
+    const float cr = __builtin_fmaf(x, ny, -(y * nx));
+    const float q = cr * inv_h;
+    const float dxy = __builtin_fmaf(x, nx, y * ny);
+    float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
+    float ry = __builtin_fmaf(ny, z, __builtin_fmaf(y, nz, x * q));
+    float rz = __builtin_fmaf(nz, z, -dxy);
+    const bool pole = (s2 == 0.0f) & !up;
+    rx = pole ? nx : rx;
+    ry = pole ? -ny : ry;
+    rz = pole ? -nz : rz;
+    const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
+    ox = rx * inv;
+    oy = ry * inv;
+    oz = rz * inv;
+}
+
+// Linear interpolation in a uniform-grid table, clamped (same operation order
+// as the oracle's or_table_).  The medium table interleaves {n, λ_sca, λ_abs, 0}.
+__device__ __forceinline__ float4 table_lerp(const float4* t, uint32_t n, float xs) {
+    const float xmax = (float)(n - 1);
+    xs = fminf(fmaxf(xs, 0.0f), xmax);
+    uint32_t i = (uint32_t)xs;
+    i = i > n - 2 ? n - 2 : i;
+    const float f = xs - (float)i;
+    const float4 lo = t[i];
+    const float4 hi = t[i + 1];
+    float4 r;
+    r.x = __builtin_fmaf(f, hi.x - lo.x, lo.x);
+    r.y = __builtin_fmaf(f, hi.y - lo.y, lo.y);
+    r.z = __builtin_fmaf(f, hi.z - lo.z, lo.z);
+    r.w = 0.0f;
+    return r;
+}
+
+// fast_linear_interp(u, knots, 0, 1) on equidistant knots (src/utils.jl:37-60).
+__device__ __forceinline__ float knot_interp(const float* knots, uint32_t n, float step_size,
+                                             float u) {
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    const float along = u / step_size;
+    const float fl = floorf(along);
+    const int64_t k = (int64_t)fl;
+    if (k >= (int64_t)n - 1) return knots[n - 1];
+    const float along_step = along - fl;
+    const float y_low = knots[k];
+    const float slope = knots[k + 1] - y_low;
+    return y_low + slope * along_step;
+}
+
+}  // namespace ppd

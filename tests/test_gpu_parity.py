@@ -1,0 +1,338 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI.
+
+Per-photon: both sides draw photon i from the same Philox stream, so the hit
+sets must agree photon by photon (see oracle/compare.py for the tolerances).
+Statistical: independent seeds, KS / χ² on time, wavelength and impact angle,
+summed weight within 3σ.  Deterministic unit checks on the device helpers.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle
+import photonprop as pp
+from oracle.compare import assert_parity, compare_hits, oracle_for
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def cone_source(n, spectrum=None, axis=(0.0, 0.0, 1.0), profile="CherenkovCone", beta=1.0):
+    return pp.PhotonSource((0.0, 0.0, 0.0), axis, 0.0, int(n),
+                           spectrum or pp.InverseSquareSpectrum(300.0, 600.0),
+                           pp.AngularEmissionProfile(profile), beta)
+
+
+# ------------------------------------------------------------ device helpers ----
+
+def test_device_philox_matches_known_answers(ctx):
+    kat = json.load(open(os.path.join(HERE, "golden", "philox_kat.json")))
+    for case in kat["cases"]:
+        ctr = np.array(case["counter"], dtype=np.uint32)
+        key = case["key"][0] | (case["key"][1] << 32)
+        out = np.zeros(4, dtype=np.uint32)
+        pp._abi.check(ctx.lib.pp_selftest_philox(ctx.handle, pp._abi.ptr(ctr), key, pp._abi.ptr(out), 1))
+        assert list(out) == case["out"], case
+    # bulk: device == oracle on 4096 counters
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2 ** 32, size=(4096, 4), dtype=np.uint64).astype(np.uint32)
+    out = np.zeros((4096, 4), dtype=np.uint32)
+    seed = 0x0123456789ABCDEF
+    pp._abi.check(ctx.lib.pp_selftest_philox(ctx.handle, pp._abi.ptr(ctr), seed, pp._abi.ptr(out), 4096))
+    for i in range(0, 4096, 97):
+        want = oracle.philox4x32_10(ctr[i], [seed & 0xFFFFFFFF, seed >> 32])
+        assert list(out[i]) == list(want)
+
+
+def test_device_rotation_kats_and_rodrigues(ctx):
+    ey = [0.0, 1.0, 0.0]
+    old = np.array([[0, 0, 1], [0, 0, -1], ey, [0.2, 0.3, 1.0]], dtype=np.float32)
+    old[3] /= np.linalg.norm(old[3])
+    new = np.array([ey, ey, ey, [-5.0, 1.0, -20.0]], dtype=np.float32)
+    new[3] /= np.linalg.norm(new[3])
+    out = np.zeros_like(old)
+    pp._abi.check(ctx.lib.pp_selftest_rotate(ctx.handle, pp._abi.ptr(old), pp._abi.ptr(new),
+                                             pp._abi.ptr(out), 4))
+    # src/tests/test_photon_prob.jl:26-30, :37-41, :43-53
+    np.testing.assert_allclose(out[0], ey, atol=1e-7)
+    np.testing.assert_allclose(out[1], [0, -1, 0], atol=1e-7)
+    np.testing.assert_allclose(out[2], [0, 0, -1], atol=1e-6)
+    assert abs(np.linalg.norm(out[3]) - 1) < 1e-6
+    # random pairs against Rodrigues (:55-64) and the FP64 oracle
+    rng = np.random.default_rng(4)
+    n = 20000
+    a = rng.normal(size=(n, 3)); a /= np.linalg.norm(a, axis=1, keepdims=True)
+    v = rng.normal(size=(n, 3)); v /= np.linalg.norm(v, axis=1, keepdims=True)
+    a32, v32 = a.astype(np.float32), v.astype(np.float32)
+    out = np.zeros_like(a32)
+    pp._abi.check(ctx.lib.pp_selftest_rotate(ctx.handle, pp._abi.ptr(a32), pp._abi.ptr(v32),
+                                             pp._abi.ptr(out), n))
+    ez = np.array([0.0, 0.0, 1.0])
+    worst = 0.0
+    for i in range(0, n, 37):
+        ai = a32[i].astype(np.float64); ai /= np.linalg.norm(ai)
+        rr = oracle.rodrigues_rotation(ez, ai, v32[i].astype(np.float64))
+        worst = max(worst, float(np.abs(out[i] - rr).max()))
+    assert worst < 5e-6, worst
+    # the angle to the old direction is the local polar angle, everywhere
+    cosang = np.einsum("ij,ij->i", out.astype(np.float64), a32.astype(np.float64))
+    assert np.abs(cosang - v32[:, 2]).max() < 5e-6
+
+
+@pytest.mark.parametrize("g", [0.92, 0.97, 0.5])
+def test_device_hg_mean_equals_g(ctx, g):
+    # North-star unit check: mean HG cosine equals g to 1e-3 (1e8 device draws),
+    # and E[cos²θ] = (1 + 2g²)/3.
+    n = 100_000_000
+    sums = np.zeros(2)
+    pp._abi.check(ctx.lib.pp_selftest_hg_moments(ctx.handle, float(g), 11, n, pp._abi.ptr(sums)))
+    mean, m2 = sums[0] / n, sums[1] / n
+    say("HG g=%g: mean %.6f  E[c^2] %.6f (want %.6f)" % (g, mean, m2, (1 + 2 * g * g) / 3))
+    assert abs(mean - g) < 1e-3
+    assert abs(m2 - (1 + 2 * g * g) / 3) < 1e-3
+
+
+def test_device_hg_per_draw_matches_oracle_and_cdf(ctx):
+    g = 0.92
+    n = 50_000
+    out = np.zeros(n, dtype=np.float32)
+    pp._abi.check(ctx.lib.pp_selftest_hg(ctx.handle, g, 3, pp._abi.ptr(out), n))
+    words = np.concatenate([oracle.philox4x32_10([i, 0, 0, 0], [3, 0]) for i in range(n // 4)])
+    ref = np.array([oracle.hg_costheta(g, oracle.uniform(int(w))) for w in words])
+    np.testing.assert_allclose(out, ref, atol=3e-6)
+    cdf = lambda mu: (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * mu) - 1 / (1 + g))
+    assert stats.kstest(out.astype(np.float64), cdf).pvalue > 0.01
+
+
+# --------------------------------------------------------- per-photon parity ----
+
+CASES = [
+    # (name, source kwargs, target centre, radius, max_steps, seed, stream)
+    ("cone_near", dict(n=300_000), (1.5, 0.0, 1.5), 0.3, 10, 1, 0),
+    ("cone_10m_90deg", dict(n=2_000_000), (10.0, 0.0, 0.0), 0.21, 10, 0, 0),
+    ("cone_axis_tilted", dict(n=400_000, axis=(0.5, -0.3, 0.81)), (2.0, 0.5, 2.0), 0.5, 10, 5, 3),
+    ("mono_many_steps", dict(n=200_000, spectrum=pp.Monochromatic(420.0)), (0.0, 3.0, 0.5), 0.5, 37, 9, 1),
+    ("uniform_sphere", dict(n=300_000, profile="UniformSphere"), (-2.0, 1.0, 0.0), 0.5, 10, 2, 0),
+    ("ref_isotropic", dict(n=300_000, profile="IsotropicEmission"), (0.0, 0.0, 3.0), 0.4, 10, 0, 0),
+    ("one_step", dict(n=300_000), (1.0, 0.0, 0.9), 0.4, 1, 4, 0),
+    ("ragged_tile", dict(n=8192 * 3 + 77), (1.2, 0.0, 1.1), 0.5, 10, 8, 0),
+    ("tiny", dict(n=5), (0.5, 0.0, 0.45), 0.3, 10, 0, 0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_per_photon_parity(ctx, case):
+    name, skw, centre, radius, max_steps, seed, stream = case
+    src = cone_source(**skw)
+    tgt = pp.DetectionSphere(centre, radius)
+    say("case", name, skw.get("n"), centre, radius, max_steps, seed, stream)
+    res = ctx.propagate(src, tgt, seed=seed, stream=stream, max_steps=max_steps)
+    ref, rsum = oracle_for(ctx, src, tgt, seed=seed, stream=stream, max_steps=max_steps)
+    cmp = compare_hits(res.hits, ref)
+    say(name, res.summary, cmp)
+    s = res.summary
+    assert s["n_sim"] == src.photons
+    assert s["n_hits"] == s["n_stored"] == res.hits.shape[0]
+    assert_parity(cmp, rsum["n_hits"])
+    flips = cmp["only_gpu"] + cmp["only_cpu"]
+    assert abs(int(s["n_steps"]) - int(rsum["n_steps"])) <= max_steps * flips
+    if rsum["n_hits"]:
+        assert s["sum_w"] == pytest.approx(rsum["sum_w"], rel=1e-4 + 2.0 * flips / rsum["n_hits"])
+    # every hit lies on the sphere surface and has a unit direction
+    if res.hits.shape[0]:
+        r = np.linalg.norm(res.hits["pos"].astype(np.float64) - np.array(centre), axis=1)
+        assert np.abs(r - radius).max() < 1e-3
+        assert np.abs(np.linalg.norm(res.hits["dir"], axis=1) - 1).max() < 1e-5
+        assert np.all(res.hits["weight"] > 0) and np.all(res.hits["weight"] <= 1)
+
+
+def test_knot_spectrum_and_reference_medium_parity():
+    # The reference's own setup: cascadia water, CherenkovSpectrum((300, 800), 20),
+    # :IsotropicEmission, g = 0.99, sphere on the z axis (src/photon_prop_cuda.jl:588-609).
+    m = pp.make_cascadia_medium_properties()
+    with pp.Context(m) as c2:
+        spec = pp.CherenkovSpectrum((300.0, 800.0), 20, m)
+        src = cone_source(500_000, spectrum=spec, profile="IsotropicEmission")
+        tgt = pp.DetectionSphere((0.0, 0.0, 3.0), 0.21)
+        res = c2.propagate(src, tgt, seed=0)
+        ref, rsum = oracle_for(c2, src, tgt, seed=0)
+        cmp = compare_hits(res.hits, ref)
+        say("reference medium", res.summary, cmp)
+        assert rsum["n_hits"] > 100
+        assert_parity(cmp, rsum["n_hits"])
+        df = pp.propagate_distance(3.0, m, 200_000)
+        assert list(df.columns) == ["tres", "initial_theta", "ref_ix", "abs_weight",
+                                    "dist_travelled", "wavelength"]
+        assert len(df) > 10
+        assert df["initial_theta"].max() <= 1.0 + 1e-6  # θ ~ U(-1, 1) rad in the reference profile
+        assert (df["tres"] > -1.0).all()
+
+
+def test_seed_and_stream_change_the_photons(ctx):
+    src = cone_source(200_000)
+    tgt = pp.DetectionSphere((1.5, 0.0, 1.5), 0.3)
+    a = ctx.propagate(src, tgt, seed=1).hits
+    b = ctx.propagate(src, tgt, seed=2).hits
+    c = ctx.propagate(src, tgt, seed=1, stream=1).hits
+    a2 = ctx.propagate(src, tgt, seed=1).hits
+    np.testing.assert_array_equal(np.sort(a["photon_id"]), np.sort(a2["photon_id"]))
+    ids = lambda h: set(h["photon_id"].tolist())
+    assert len(ids(a) & ids(b)) < 0.2 * len(ids(a))
+    assert len(ids(a) & ids(c)) < 0.2 * len(ids(a))
+
+
+# ------------------------------------------------------ closed forms on device ----
+
+class ConstMedium(pp.medium.MediumProperties):
+    def __init__(self, n=1.34, abs_len=20.0, sca_len=1e30, g=0.92):
+        self.n, self.a, self.s, self.hg_g = n, abs_len, sca_len, g
+        self.table_range = (300.0, 600.0)
+        self.table_points = 3
+
+    def refractive_index(self, w):
+        return np.full_like(np.asarray(w, dtype=float), self.n)
+
+    def absorption_length(self, w):
+        return np.full_like(np.asarray(w, dtype=float), self.a)
+
+    def scattering_length(self, w):
+        return np.full_like(np.asarray(w, dtype=float), self.s)
+
+
+def test_device_survival_exp_minus_d_over_abs():
+    # North-star unit check: survival equals exp(-d/λ_abs) to 1e-3.
+    n, d, r, la = 1.34, 12.0, 0.21, 25.0
+    with pp.Context(ConstMedium(n=n, abs_len=la)) as c:
+        src = cone_source(1_000_000, spectrum=pp.Monochromatic(450.0), beta=1.0 / n)
+        res = c.propagate(src, pp.DetectionSphere((0.0, 0.0, d), r))
+    s = res.summary
+    assert s["n_hits"] == src.photons
+    surv = s["sum_w"] / src.photons
+    say("survival", surv, math.exp(-(d - r) / la))
+    assert abs(surv - math.exp(-(d - r) / la)) < 1e-3
+    np.testing.assert_allclose(res.hits["dist"], d - r, rtol=1e-6)
+
+
+def test_device_no_scatter_cone_fraction():
+    n, d, r = 1.34, 5.0, 0.4
+    thc = math.acos(1 / n)
+    alpha = thc + 0.03
+    with pp.Context(ConstMedium(n=n)) as c:
+        src = cone_source(4_000_000, spectrum=pp.Monochromatic(450.0))
+        res = c.propagate(src, pp.DetectionSphere((d * math.sin(alpha), 0, d * math.cos(alpha)), r))
+    rho = math.asin(r / d)
+    x = (math.cos(rho) - math.cos(thc) * math.cos(alpha)) / (math.sin(thc) * math.sin(alpha))
+    p = math.acos(x) / math.pi
+    N = src.photons
+    k = res.summary["n_hits"]
+    assert abs(k - N * p) < 5 * math.sqrt(N * p * (1 - p)), (k, N * p)
+
+
+def test_device_unscattered_fraction_and_unit_weight():
+    n, d, r, ls = 1.34, 6.0, 0.3, 8.0
+    with pp.Context(ConstMedium(n=n, abs_len=1e30, sca_len=ls)) as c:
+        src = cone_source(4_000_000, spectrum=pp.Monochromatic(450.0), beta=1.0 / n)
+        res = c.propagate(src, pp.DetectionSphere((0.0, 0.0, d), r), max_steps=1)
+    p = math.exp(-(d - r) / ls)
+    N = src.photons
+    k = res.summary["n_hits"]
+    assert abs(k - N * p) < 5 * math.sqrt(N * p * (1 - p))
+    assert np.all(res.hits["weight"] == 1.0)
+    assert res.summary["n_steps"] == N
+
+
+# ---------------------------------------------------------------- statistics ----
+
+GEOMS = [(3.0, 45.0), (10.0, 45.0), (30.0, 45.0), (1.0, 0.0), (5.0, 90.0)]
+
+
+@pytest.mark.parametrize("dist,angle", GEOMS)
+def test_statistical_equivalence_independent_seeds(ctx, dist, angle):
+    """Config physics with the target at `dist` m, `angle` degrees from the
+    emitter axis: GPU (2e8 photons, seed 1) vs FP64 oracle (up to 6e7
+    photons, seed 2).  Σw/N within 3σ; KS p > 0.01 on arrival time, wavelength and
+    impact angle."""
+    a = math.radians(angle)
+    tgt = pp.DetectionSphere((dist * math.sin(a), 0.0, dist * math.cos(a)), 0.21)
+    n_gpu = 200_000_000
+    gsrc = cone_source(n_gpu)
+    g = ctx.propagate(gsrc, tgt, seed=1)
+    # size the CPU run for >= ~3000 hits, capped by time
+    rate = g.summary["n_hits"] / n_gpu
+    n_cpu = int(min(max(3000 / max(rate, 1e-9), 1e6), 6e7))
+    csrc = cone_source(n_cpu)
+    ch, cs = oracle_for(ctx, csrc, tgt, seed=2, precision=64)
+    say("geom", dist, angle, "gpu hits", g.summary["n_hits"], "cpu n", n_cpu, "cpu hits", cs["n_hits"])
+    assert cs["n_hits"] > 150 and g.summary["n_hits"] > 150
+    # per-photon weighted yield: the GPU's error is negligible at 1e8; the CPU's
+    # variance from the per-hit weights
+    w = ch["weight"].astype(np.float64)
+    mean_c = cs["sum_w"] / n_cpu
+    sd_c = math.sqrt(max((w ** 2).sum() / n_cpu - mean_c ** 2, 0.0) / n_cpu)
+    wg = g.hits["weight"].astype(np.float64)
+    mean_g = g.summary["sum_w"] / n_gpu
+    sd_g = math.sqrt(max((wg ** 2).sum() / n_gpu - mean_g ** 2, 0.0) / n_gpu)
+    z = (mean_g - mean_c) / math.hypot(sd_c, sd_g)
+    say("yield gpu %.4e cpu %.4e z=%.2f" % (mean_g, mean_c, z))
+    assert abs(z) < 3.0
+    cdir = np.array(tgt.position) / np.linalg.norm(tgt.position)
+
+    def impact(h):
+        n = (h["pos"].astype(np.float64) - np.array(tgt.position)) / tgt.radius
+        return np.einsum("ij,ij->i", -h["dir"].astype(np.float64), n)
+
+    for name, fg, fc in (("time", g.hits["time"], ch["time"]),
+                         ("wavelength", g.hits["wavelength"], ch["wavelength"]),
+                         ("impact", impact(g.hits), impact(ch))):
+        p = stats.ks_2samp(np.asarray(fg, np.float64), np.asarray(fc, np.float64)).pvalue
+        say("KS", name, p)
+        assert p > 0.01, (name, p)
+    # χ² on the arrival-time histogram (weighted by counts, equal-probability bins)
+    edges = np.quantile(ch["time"], np.linspace(0, 1, 11))
+    edges[0], edges[-1] = -np.inf, np.inf
+    cg = np.histogram(g.hits["time"], edges)[0]
+    cc = np.histogram(ch["time"], edges)[0]
+    chi2, p, _, _ = stats.chi2_contingency(np.vstack([cg, cc]))
+    say("chi2 time", chi2, p)
+    assert p > 0.01
+
+
+# ------------------------------------------------------------- failure paths ----
+
+def test_overflow_counts_exact(ctx):
+    src = cone_source(300_000)
+    tgt = pp.DetectionSphere((1.5, 0.0, 1.5), 0.3)
+    full = ctx.propagate(src, tgt, seed=1)
+    with pytest.raises(pp.PhotonPropError) as e:
+        ctx.propagate(src, tgt, seed=1, hits_cap=5)
+    assert e.value.code == pp._abi.PP_ERR_OVERFLOW
+    part = ctx.propagate(src, tgt, seed=1, hits_cap=5, allow_overflow=True)
+    assert part.summary["n_hits"] == full.summary["n_hits"]
+    assert part.summary["sum_w"] == full.summary["sum_w"]  # sums are exact, bitwise
+    assert part.summary["n_stored"] == 5 and part.hits.shape[0] == 5
+    assert part.summary["status"] == pp._abi.PP_ERR_OVERFLOW
+    assert set(part.hits["photon_id"].tolist()) <= set(full.hits["photon_id"].tolist())
+
+
+def test_invalid_inputs_raise(ctx):
+    tgt = pp.DetectionSphere((1.0, 0.0, 0.0), 0.3)
+    with pytest.raises(pp.PhotonPropError, match="radius"):
+        ctx.propagate(cone_source(10), pp.DetectionSphere((1.0, 0.0, 0.0), 0.0))
+    with pytest.raises(pp.PhotonPropError, match="non-finite"):
+        ctx.propagate(pp.PhotonSource((float("nan"), 0, 0), (0, 0, 1), 0.0, 10,
+                                      pp.Monochromatic(400.0)), tgt)
+    with pytest.raises(pp.PhotonPropError, match="Cherenkov"):
+        ctx.propagate(cone_source(10, beta=0.5), tgt)
+    with pytest.raises(pp.PhotonPropError, match="max_steps"):
+        ctx.propagate(cone_source(10), tgt, max_steps=0)
+    # zero photons: an empty, exact result
+    res = ctx.propagate(cone_source(0), tgt)
+    assert res.summary["n_sim"] == 0 and res.summary["n_hits"] == 0 and res.hits.shape[0] == 0
