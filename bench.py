@@ -92,7 +92,21 @@ def main():
     ap.add_argument("--photons", type=float, default=1e9, help="photons per geometry")
     ap.add_argument("--cpu-sample", type=float, default=1e6,
                     help="photons per geometry for the CPU baseline (0 skips it)")
+    ap.add_argument("--copy-probe", type=float, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter "
+                         "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = int(args.copy_probe) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "dispatches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU; --gpus %d is not supported (the path shards by photon-id "
               "range across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -106,8 +120,8 @@ def main():
     reqs = workload_requests(pp, n_per)
     spectrum = pp.InverseSquareSpectrum(300.0, 600.0)
     ctx = pp.Context(pp.SeawaterConfigMedium(), device=0)
-    # ~0.26 hits per 1e3 photons over the 64 geometries; keep 2x headroom
-    cap = int(max(1 << 20, 6.0e-4 * n_per * len(reqs)))
+    # ~4.1e-3 hits per photon averaged over the 64 geometries; keep 2x headroom
+    cap = int(max(1 << 20, 8.2e-3 * n_per * len(reqs)))
     d_hits = torch.empty((cap, HIT_BYTES), dtype=torch.uint8, device="cuda:0")
     d_off = torch.zeros(len(reqs) + 1, dtype=torch.int64, device="cuda:0")
     d_sum = torch.zeros((len(reqs), 64), dtype=torch.uint8, device="cuda:0")

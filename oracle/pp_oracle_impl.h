@@ -165,6 +165,7 @@ static inline void FN(or_init_direction_)(const pp_emitter* em, const REAL* axis
         ct = R_FMA((REAL)2, u_theta, (REAL)-1);
     else
         ct = (REAL)1 / (n_phase * (REAL)em->beta);
+    if (ct > (REAL)1) ct = (REAL)1; /* n·β = 1 within rounding: emission along the axis */
     REAL st = R_SQRT(R_FMA(-ct, ct, (REAL)1));
     REAL local[3] = {cp * st, sp * st, ct};
     FN(or_rotate_to_axis)(axis, local, dir);

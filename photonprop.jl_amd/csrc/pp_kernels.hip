@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) propagate_kernel(KParams P) {
                 } else {
                     const float ct = R.profile == PP_EMIT_ISOTROPIC
                                          ? __builtin_fmaf(2.0f, u_th, -1.0f)
-                                         : 1.0f / (n_phase * R.beta);
+                                         : fminf(1.0f / (n_phase * R.beta), 1.0f);
                     const float st = sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
                     ppd::rotate_to_axis(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st, ct, dx,
                                         dy, dz);
@@ -675,7 +675,7 @@ int make_devreq(const pp_request* q, uint32_t idx, DevReq* d) {
 int check_cone(const pp_ctx* c, const pp_request* reqs, uint32_t n_req, float n_min) {
     for (uint32_t i = 0; i < n_req; ++i) {
         if (reqs[i].emitter.profile != PP_EMIT_CHERENKOV_CONE) continue;
-        if (!((double)n_min * (double)reqs[i].emitter.beta >= 1.0))
+        if (!((double)n_min * (double)reqs[i].emitter.beta >= 1.0 - 1e-6))
             return fail(PP_ERR_INVALID_ARG,
                         "request %u: no Cherenkov emission (n_min=%g, beta=%g, n*beta < 1)", i,
                         (double)n_min, (double)reqs[i].emitter.beta);
