@@ -84,6 +84,75 @@ def cpu_baseline(pp, photons_per_geometry_sample):
     }
 
 
+def bench_batched(pp, ctx, spectrum, reps=3):
+    """4096 source-target pairs of uneven size in one persistent launch."""
+    from photonprop import workloads
+
+    reqs = workloads.batched_pairs(4096, seed=0)
+    ctx.propagate_batch(reqs, spectrum)  # warm-up
+    t0 = time.perf_counter()
+    ks = []
+    for _ in range(reps):
+        hits, off, sums = ctx.propagate_batch(reqs, spectrum)
+        ks.append(ctx.last_kernel_ms())
+    dt = (time.perf_counter() - t0) / reps
+    steps = int(sums["n_steps"].sum())
+    return {
+        "pairs": 4096,
+        "photons": int(reqs["n_photons"].sum()),
+        "photon_steps": steps,
+        "hits": int(sums["n_hits"].sum()),
+        "kernel_ms": sum(ks) / len(ks),
+        "call_ms_incl_d2h": 1e3 * dt,
+        "photon_steps_per_s_kernel": steps / (1e-3 * sum(ks) / len(ks)),
+        "photon_steps_per_s_call": steps / dt,
+    }
+
+
+def bench_serving(pp, ctx, spectrum, n_req):
+    """Poisson arrivals of requests (budgets log-uniform 1e4..1e8 photons,
+    random pose, target 1-150 m) into the serving loop at several rates,
+    against one pp_propagate launch per request (closed loop, same requests)."""
+    from photonprop import workloads
+
+    out = {"requests_per_rate": n_req, "rates": {}}
+    # per-request launch: every request alone, back to back
+    _, reqs = workloads.serving_requests(n_req, 1000.0, seed=11)
+    t0 = time.perf_counter()
+    steps = 0
+    for r in reqs:
+        src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0, int(r["n_photons"]), spectrum)
+        res = ctx.propagate(src, pp.DetectionSphere(tuple(r["center"]), float(r["radius"])),
+                            seed=int(r["seed"]), stream=int(r["stream"]), hits_cap=0,
+                            allow_overflow=True)
+        steps += res.summary["n_steps"]
+    dt = time.perf_counter() - t0
+    out["per_request_launch"] = {"requests_per_s": n_req / dt,
+                                 "photons_per_s": float(reqs["n_photons"].sum()) / dt,
+                                 "photon_steps_per_s": steps / dt}
+    with pp.Server(ctx, spectrum, max_slots=1024, max_tiles_per_request=16384,
+                   hit_pool_records=1 << 20) as srv:
+        for rate in (2000.0, 5000.0, 10000.0, 20000.0):
+            arr, reqs = workloads.serving_requests(n_req, rate, seed=11)
+            res = pp.run_open_loop(srv, arr, reqs, hits_cap=0)
+            lat = res["latency_us"]
+            out["rates"][str(int(rate))] = {
+                "p50_ms": float(np.percentile(lat, 50)) / 1e3,
+                "p99_ms": float(np.percentile(lat, 99)) / 1e3,
+                "requests_per_s": n_req / res["makespan_s"],
+                "photons_per_s": res["photons"] / res["makespan_s"],
+                "photon_steps_per_s": res["steps"] / res["makespan_s"],
+            }
+            print("serving rate %d: %s" % (rate, out["rates"][str(int(rate))]), file=sys.stderr,
+                  flush=True)
+        out["server_stats"] = srv.stats()
+    base = out["per_request_launch"]["requests_per_s"]
+    ok = [v["requests_per_s"] for v in out["rates"].values() if v["p99_ms"] < 20.0]
+    out["best_requests_per_s_at_p99_under_20ms"] = max(ok) if ok else None
+    out["speedup_vs_per_request_launch"] = (max(ok) / base) if ok else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +161,10 @@ def main():
     ap.add_argument("--photons", type=float, default=1e9, help="photons per geometry")
     ap.add_argument("--cpu-sample", type=float, default=1e6,
                     help="photons per geometry for the CPU baseline (0 skips it)")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="also measure the 4096-pair batch and the serving loop (0 skips)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests per arrival rate in the serving measurement")
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter "
                          "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -202,6 +275,11 @@ def main():
                     % FLOPS_PER_STEP,
         },
     }
+    if args.extras:
+        out["batched_4096"] = bench_batched(pp, ctx, spectrum)
+        print("batched done", file=sys.stderr, flush=True)
+        out["serving"] = bench_serving(pp, ctx, spectrum, args.serve_requests)
+        print("serving done", file=sys.stderr, flush=True)
     if args.cpu_sample > 0:
         cb = cpu_baseline(pp, int(args.cpu_sample))
         out["cpu_baseline"] = cb

@@ -213,6 +213,60 @@ int pp_synchronize(pp_ctx* ctx);
  * context, measured with HIP events on the context's stream. */
 int pp_last_kernel_ms(pp_ctx* ctx, float* ms);
 
+/* ---- serving loop: continuous batching into a persistent kernel ------------ */
+/* No counterpart in the reference (which launches one compile-time-specialised
+ * kernel per call, src/photon_prop_cuda.jl:607).  Requests are queued on the
+ * host, admitted when a device slot and room in the device hit pool (the HBM
+ * budget) are free, and their photon tiles are published to a persistent
+ * kernel that keeps running while work arrives.  Each request's summary and
+ * hits are bitwise the same as pp_propagate gives for it alone.  All server
+ * calls are thread-safe. */
+
+#define PP_TILE_PHOTONS 8192
+
+typedef struct pp_server_config {
+    uint32_t max_slots;             /* requests on the device at once, 1..4096 */
+    uint32_t max_tiles_per_request; /* photons per request <= this * PP_TILE_PHOTONS */
+    uint64_t hit_pool_records;      /* device hit-record pool shared by admitted requests */
+    uint32_t idle_exit_us;          /* persistent kernel exits after this long without work
+                                       (relaunched on demand); 0 -> 20000 */
+    uint32_t reserved;
+} pp_server_config;
+
+typedef struct pp_server_stats {
+    uint64_t submitted, completed, in_flight, pending;
+    uint64_t kernel_launches;       /* persistent-kernel launches so far */
+    uint64_t pool_records_free;
+    uint64_t tiles_published;
+} pp_server_stats;
+
+typedef struct pp_server pp_server;
+
+/* Create a server on ctx's device for requests drawn from `spectrum`. */
+int pp_server_create(pp_ctx* ctx, const pp_spectrum* spectrum, const pp_server_config* cfg,
+                     pp_server** out);
+
+/* Queue a request; never blocks.  hits_cap records of the device pool are
+ * reserved for it while it is admitted (0: summary only).  Fails with
+ * PP_ERR_INVALID_ARG if the request can never fit (too many photons or a
+ * hits_cap above the pool). */
+int pp_server_submit(pp_server* srv, const pp_request* request, uint64_t hits_cap,
+                     uint64_t* ticket_out);
+
+/* Wait up to timeout_ms (0: poll) for a ticket; *done_out = 1 when finished. */
+int pp_server_wait(pp_server* srv, uint64_t ticket, uint32_t timeout_ms, int* done_out);
+
+/* Results of a finished ticket: summary, hits (up to hits_cap records, the
+ * count in *n_hits_out) and the submit-to-completion latency.  Frees the
+ * ticket and its pool records. */
+int pp_server_result(pp_server* srv, uint64_t ticket, pp_summary* summary, pp_hit* hits_out,
+                     uint64_t hits_cap, uint64_t* n_hits_out, double* latency_us);
+
+int pp_server_stats_get(pp_server* srv, pp_server_stats* out);
+
+/* Drain: stop admitting, let the kernel finish in-flight work and exit. */
+int pp_server_destroy(pp_server* srv);
+
 /* ---- device self-tests of the kernel's own helper functions ---------------- */
 /* The reference checks its device helpers through tiny test kernels
  * (apply_rot, src/tests/test_photon_prob.jl:32-41).  These run the helpers the

@@ -22,12 +22,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/photonprop.h"
@@ -108,27 +114,236 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+struct TileCtx {
+    const float4* table;  // LDS copy of the medium table
+    const float* knots;   // LDS copy of the spectrum knots
+    uint32_t n_points;
+    float lmin, inv_dl;
+    ppd::HG hg;
+    SpecParams spec;
+};
+
+// Photons [first, first + count) of request R on one 256-lane block: emit,
+// step, append hits (record field `request` = req_index), and reduce the
+// tile's counts and FP64 sums in a fixed order.  The result is valid in
+// thread 0 only.  Shared by the batch kernel and the serving kernel, so a
+// request's tile partials are the same whichever kernel ran them.
+__device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& R,
+                                                uint32_t req_index, uint64_t first,
+                                                uint32_t count, pp_hit* hits, uint64_t hits_cap,
+                                                unsigned long long* hit_cursor,
+                                                double (*s_red)[3], uint32_t (*s_redi)[2]) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const uint32_t rounds = (count + kBlock - 1) / kBlock;
+    const uint32_t max_steps = R.max_steps;
+    const float r2 = R.r2;
+    const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
+
+    uint32_t acc_steps = 0, acc_hits = 0;
+    double acc_w = 0.0, acc_wt = 0.0, acc_wt2 = 0.0;
+
+    for (uint32_t k = 0; k < rounds; ++k) {
+        const uint32_t local = k * kBlock + tid;
+        const bool active = local < count;
+        const uint64_t pid = first + local;
+        const uint32_t id_lo = (uint32_t)pid, id_hi = (uint32_t)(pid >> 32);
+
+        // ---- emission (src/photon_prop_cuda.jl:308-316) ----
+        const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+        const float u_l = ppd::u01(w0.x);
+        float lambda;
+        if (T.spec.kind == PP_SPEC_MONO) {
+            lambda = T.spec.mono;
+        } else if (T.spec.kind == PP_SPEC_KNOTS) {
+            lambda = ppd::knot_interp(T.knots, T.spec.n_knots, T.spec.knot_step, u_l);
+        } else {
+            lambda = 1.0f / __builtin_fmaf(-u_l, T.spec.d, T.spec.a);
+        }
+        const float4 med = ppd::table_lerp(T.table, T.n_points, (lambda - T.lmin) * T.inv_dl);
+        const float n_phase = med.x;
+        const float sca_ln2 = med.y * kLn2;
+        const float abs_len = med.z;
+
+        float dx, dy, dz;
+        {
+            const float u_phi = ppd::u01(w0.y);
+            const float sp = __builtin_amdgcn_sinf(u_phi);
+            const float cp = __builtin_amdgcn_cosf(u_phi);
+            const float u_th = ppd::u01(w0.z);
+            if (R.profile == PP_EMIT_REF_ISOTROPIC) {
+                const float theta = __builtin_fmaf(u_th, 2.0f, -1.0f);
+                const float st = __builtin_amdgcn_sinf(theta * kInv2Pi);
+                const float ct = __builtin_amdgcn_cosf(theta * kInv2Pi);
+                dx = cp * st;
+                dy = sp * st;
+                dz = ct;
+            } else {
+                const float ct = R.profile == PP_EMIT_ISOTROPIC
+                                     ? __builtin_fmaf(2.0f, u_th, -1.0f)
+                                     : fminf(1.0f / (n_phase * R.beta), 1.0f);
+                const float st = sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
+                ppd::rotate_to_axis(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st, ct, dx,
+                                    dy, dz);
+            }
+        }
+        const float idx = dx, idy = dy, idz = dz;
+        float px = R.pos[0], py = R.pos[1], pz = R.pos[2];
+        float dist = 0.0f;
+        bool alive = active;
+        bool hit = false;
+        uint32_t nsteps = 0;
+
+        // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
+#define PP_STEP(WA, WB, WC)                                                              \
+    if (alive) {                                                                        \
+        const float u1 = ppd::u01(WA);                                                  \
+        float s = -__builtin_amdgcn_logf(u1) * sca_ln2;                                 \
+        const float rx = px - cx, ry = py - cy, rz = pz - cz;                           \
+        const float a = __builtin_fmaf(dz, rz, __builtin_fmaf(dy, ry, dx * rx));        \
+        const float pp = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));       \
+        const float b = __builtin_fmaf(a, a, r2 - pp);                                  \
+        ++nsteps;                                                                       \
+        bool h = false;                                                                 \
+        if (b >= 0.0f) {                                                                \
+            const float d = -a - sqrtf(b);                                              \
+            if (d > 0.0f && d < s) {                                                    \
+                h = true;                                                               \
+                s = d;                                                                  \
+            }                                                                           \
+        }                                                                               \
+        px = __builtin_fmaf(s, dx, px);                                                 \
+        py = __builtin_fmaf(s, dy, py);                                                 \
+        pz = __builtin_fmaf(s, dz, pz);                                                 \
+        dist += s;                                                                      \
+        if (h) {                                                                        \
+            hit = true;                                                                 \
+            alive = false;                                                              \
+        } else {                                                                        \
+            const float ct = ppd::hg_costheta(T.hg, ppd::u01(WB));                      \
+            const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));     \
+            const float uph = ppd::u01(WC);                                             \
+            const float sph = __builtin_amdgcn_sinf(uph);                               \
+            const float cph = __builtin_amdgcn_cosf(uph);                               \
+            ppd::rotate_to_axis(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);        \
+        }                                                                               \
+    }
+
+        for (uint32_t s0 = 0; s0 < max_steps; s0 += 4) {
+            if (__ballot(alive) == 0) break;
+            const uint32_t blk = 1u + 3u * (s0 >> 2);
+            const ppd::U4 A = ppd::philox10(blk, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(A.x, A.y, A.z)
+            if (s0 + 1 >= max_steps || __ballot(alive) == 0) break;
+            const ppd::U4 B =
+                ppd::philox10(blk + 1u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(A.w, B.x, B.y)
+            if (s0 + 2 >= max_steps || __ballot(alive) == 0) break;
+            const ppd::U4 C =
+                ppd::philox10(blk + 2u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(B.z, B.w, C.x)
+            if (s0 + 3 >= max_steps) break;
+            PP_STEP(C.y, C.z, C.w)
+        }
+#undef PP_STEP
+        acc_steps += nsteps;
+
+        // ---- hit append: one atomic per wave with any hit ----
+        const uint64_t mask = __ballot(hit);
+        if (mask != 0) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(hit_cursor, (unsigned long long)__popcll(mask));
+            base = __shfl(base, 0, 64);
+            if (hit) {
+                const float e = -dist / abs_len;
+                const float wgt = __builtin_amdgcn_exp2f(e * kLog2e);
+                const float t = R.t0 + dist * n_phase / (float)kCVacuum;
+                acc_hits += 1;
+                acc_w += (double)wgt;
+                acc_wt += (double)wgt * (double)t;
+                acc_wt2 += (double)wgt * (double)t * (double)t;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                const unsigned long long slot = base + rank;
+                if (slot < hits_cap) {
+                    float4* o = reinterpret_cast<float4*>(hits + slot);
+                    o[0] = make_float4(px, py, pz, t);
+                    o[1] = make_float4(dx, dy, dz, lambda);
+                    o[2] = make_float4(idx, idy, idz, wgt);
+                    uint4 tail;
+                    tail.x = __float_as_uint(dist);
+                    tail.y = req_index;
+                    tail.z = id_lo;
+                    tail.w = id_hi;
+                    reinterpret_cast<uint4*>(o)[3] = tail;
+                }
+            }
+        }
+    }
+
+    // ---- tile partial: fixed-order reduction ----
+    const uint32_t ws = wave_sum_u32(acc_steps);
+    const uint32_t wh = wave_sum_u32(acc_hits);
+    const double w1 = wave_sum_f64(acc_w);
+    const double w2 = wave_sum_f64(acc_wt);
+    const double w3 = wave_sum_f64(acc_wt2);
+    if (lane == 0) {
+        s_redi[wave][0] = ws;
+        s_redi[wave][1] = wh;
+        s_red[wave][0] = w1;
+        s_red[wave][1] = w2;
+        s_red[wave][2] = w3;
+    }
+    __syncthreads();
+    TilePartial tp;
+    tp.n_steps = 0;
+    tp.n_hits = 0;
+    tp.sum_w = 0.0;
+    tp.sum_wt = 0.0;
+    tp.sum_wt2 = 0.0;
+    if (tid == 0) {
+        for (int q = 0; q < kWaves; ++q) {
+            tp.n_steps += s_redi[q][0];
+            tp.n_hits += s_redi[q][1];
+            tp.sum_w += s_red[q][0];
+            tp.sum_wt += s_red[q][1];
+            tp.sum_wt2 += s_red[q][2];
+        }
+    }
+    return tp;
+}
+
+// Stage the medium table and knots into LDS.
+__device__ __forceinline__ TileCtx stage_tables(float4* s_table, const float4* table,
+                                                uint32_t n_points, const float* knots,
+                                                const SpecParams& spec, float lmin, float inv_dl,
+                                                const ppd::HG& hg) {
+    float* s_knots = reinterpret_cast<float*>(s_table + n_points);
+    for (uint32_t i = threadIdx.x; i < n_points; i += kBlock) s_table[i] = table[i];
+    if (spec.kind == PP_SPEC_KNOTS)
+        for (uint32_t i = threadIdx.x; i < spec.n_knots; i += kBlock) s_knots[i] = knots[i];
+    TileCtx T;
+    T.table = s_table;
+    T.knots = s_knots;
+    T.n_points = n_points;
+    T.lmin = lmin;
+    T.inv_dl = inv_dl;
+    T.hg = hg;
+    T.spec = spec;
+    return T;
+}
+
 __global__ void __launch_bounds__(kBlock) propagate_kernel(KParams P) {
     extern __shared__ float4 s_table[];  // n_points float4, then knots
     __shared__ uint32_t s_tile;
     __shared__ double s_red[kWaves][3];
     __shared__ uint32_t s_redi[kWaves][2];
-
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    const uint32_t wave = tid >> 6;
-    float* s_knots = reinterpret_cast<float*>(s_table + P.n_points);
-
-    for (uint32_t i = tid; i < P.n_points; i += kBlock) s_table[i] = P.table[i];
-    if (P.spec.kind == PP_SPEC_KNOTS)
-        for (uint32_t i = tid; i < P.spec.n_knots; i += kBlock) s_knots[i] = P.knots[i];
-
-    const ppd::HG hg = P.hg;
-    const SpecParams spec = P.spec;
-
+    const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
+                                   P.inv_dl, P.hg);
     for (;;) {
-        __syncthreads();  // tables staged / previous tile's s_tile consumed
-        if (tid == 0) s_tile = atomicAdd(P.tile_counter, 1u);
+        __syncthreads();  // tables staged / previous tile's s_tile and s_red consumed
+        if (threadIdx.x == 0) s_tile = atomicAdd(P.tile_counter, 1u);
         __syncthreads();
         const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
         if ((uint64_t)tile >= P.n_tiles) break;
@@ -144,198 +359,21 @@ __global__ void __launch_bounds__(kBlock) propagate_kernel(KParams P) {
         const uint64_t first = ((uint64_t)tile - P.tile_start[r]) * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
-        const uint32_t rounds = (count + kBlock - 1) / kBlock;
-        const uint32_t max_steps = R.max_steps;
-        const float r2 = R.r2;
-        const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
-
-        uint32_t acc_steps = 0, acc_hits = 0;
-        double acc_w = 0.0, acc_wt = 0.0, acc_wt2 = 0.0;
-
-        for (uint32_t k = 0; k < rounds; ++k) {
-            const uint32_t local = k * kBlock + tid;
-            const bool active = local < count;
-            const uint64_t pid = first + local;
-            const uint32_t id_lo = (uint32_t)pid, id_hi = (uint32_t)(pid >> 32);
-
-            // ---- emission (src/photon_prop_cuda.jl:308-316) ----
-            const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            const float u_l = ppd::u01(w0.x);
-            float lambda;
-            if (spec.kind == PP_SPEC_MONO) {
-                lambda = spec.mono;
-            } else if (spec.kind == PP_SPEC_KNOTS) {
-                lambda = ppd::knot_interp(s_knots, spec.n_knots, spec.knot_step, u_l);
-            } else {
-                lambda = 1.0f / __builtin_fmaf(-u_l, spec.d, spec.a);
-            }
-            const float4 med = ppd::table_lerp(s_table, P.n_points, (lambda - P.lmin) * P.inv_dl);
-            const float n_phase = med.x;
-            const float sca_ln2 = med.y * kLn2;
-            const float abs_len = med.z;
-
-            float dx, dy, dz;
-            {
-                const float u_phi = ppd::u01(w0.y);
-                const float sp = __builtin_amdgcn_sinf(u_phi);
-                const float cp = __builtin_amdgcn_cosf(u_phi);
-                const float u_th = ppd::u01(w0.z);
-                if (R.profile == PP_EMIT_REF_ISOTROPIC) {
-                    const float theta = __builtin_fmaf(u_th, 2.0f, -1.0f);
-                    const float st = __builtin_amdgcn_sinf(theta * kInv2Pi);
-                    const float ct = __builtin_amdgcn_cosf(theta * kInv2Pi);
-                    dx = cp * st;
-                    dy = sp * st;
-                    dz = ct;
-                } else {
-                    const float ct = R.profile == PP_EMIT_ISOTROPIC
-                                         ? __builtin_fmaf(2.0f, u_th, -1.0f)
-                                         : fminf(1.0f / (n_phase * R.beta), 1.0f);
-                    const float st = sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
-                    ppd::rotate_to_axis(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st, ct, dx,
-                                        dy, dz);
-                }
-            }
-            const float idx = dx, idy = dy, idz = dz;
-            float px = R.pos[0], py = R.pos[1], pz = R.pos[2];
-            float dist = 0.0f;
-            bool alive = active;
-            bool hit = false;
-            uint32_t nsteps = 0;
-
-            // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
-#define PP_STEP(WA, WB, WC)                                                              \
-    if (alive) {                                                                         \
-        const float u1 = ppd::u01(WA);                                                   \
-        float s = -__builtin_amdgcn_logf(u1) * sca_ln2;                                  \
-        const float rx = px - cx, ry = py - cy, rz = pz - cz;                            \
-        const float a = __builtin_fmaf(dz, rz, __builtin_fmaf(dy, ry, dx * rx));         \
-        const float pp = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));        \
-        const float b = __builtin_fmaf(a, a, r2 - pp);                                   \
-        ++nsteps;                                                                        \
-        bool h = false;                                                                  \
-        if (b >= 0.0f) {                                                                 \
-            const float d = -a - sqrtf(b);                                               \
-            if (d > 0.0f && d < s) {                                                     \
-                h = true;                                                                \
-                s = d;                                                                   \
-            }                                                                            \
-        }                                                                                \
-        px = __builtin_fmaf(s, dx, px);                                                  \
-        py = __builtin_fmaf(s, dy, py);                                                  \
-        pz = __builtin_fmaf(s, dz, pz);                                                  \
-        dist += s;                                                                       \
-        if (h) {                                                                         \
-            hit = true;                                                                  \
-            alive = false;                                                               \
-        } else {                                                                         \
-            const float ct = ppd::hg_costheta(hg, ppd::u01(WB));                         \
-            const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));      \
-            const float uph = ppd::u01(WC);                                              \
-            const float sph = __builtin_amdgcn_sinf(uph);                                \
-            const float cph = __builtin_amdgcn_cosf(uph);                                \
-            ppd::rotate_to_axis(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);        \
-        }                                                                                \
-    }
-
-            for (uint32_t s0 = 0; s0 < max_steps; s0 += 4) {
-                if (__ballot(alive) == 0) break;
-                const uint32_t blk = 1u + 3u * (s0 >> 2);
-                const ppd::U4 A = ppd::philox10(blk, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-                PP_STEP(A.x, A.y, A.z)
-                if (s0 + 1 >= max_steps || __ballot(alive) == 0) break;
-                const ppd::U4 B =
-                    ppd::philox10(blk + 1u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-                PP_STEP(A.w, B.x, B.y)
-                if (s0 + 2 >= max_steps || __ballot(alive) == 0) break;
-                const ppd::U4 C =
-                    ppd::philox10(blk + 2u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-                PP_STEP(B.z, B.w, C.x)
-                if (s0 + 3 >= max_steps) break;
-                PP_STEP(C.y, C.z, C.w)
-            }
-#undef PP_STEP
-            acc_steps += nsteps;
-
-            // ---- hit append: one atomic per wave with any hit ----
-            const uint64_t mask = __ballot(hit);
-            if (mask != 0) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(P.hit_cursor, (unsigned long long)__popcll(mask));
-                base = __shfl(base, 0, 64);
-                if (hit) {
-                    const float e = -dist / abs_len;
-                    const float wgt = __builtin_amdgcn_exp2f(e * kLog2e);
-                    const float t = R.t0 + dist * n_phase / (float)kCVacuum;
-                    acc_hits += 1;
-                    acc_w += (double)wgt;
-                    acc_wt += (double)wgt * (double)t;
-                    acc_wt2 += (double)wgt * (double)t * (double)t;
-                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                        (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                    const unsigned long long slot = base + rank;
-                    if (slot < P.hits_cap) {
-                        float4* o = reinterpret_cast<float4*>(P.hits + slot);
-                        o[0] = make_float4(px, py, pz, t);
-                        o[1] = make_float4(dx, dy, dz, lambda);
-                        o[2] = make_float4(idx, idy, idz, wgt);
-                        uint4 tail;
-                        tail.x = __float_as_uint(dist);
-                        tail.y = r;
-                        tail.z = id_lo;
-                        tail.w = id_hi;
-                        reinterpret_cast<uint4*>(o)[3] = tail;
-                    }
-                }
-            }
-        }
-
-        // ---- tile partial: fixed-order reduction ----
-        const uint32_t ws = wave_sum_u32(acc_steps);
-        const uint32_t wh = wave_sum_u32(acc_hits);
-        const double w1 = wave_sum_f64(acc_w);
-        const double w2 = wave_sum_f64(acc_wt);
-        const double w3 = wave_sum_f64(acc_wt2);
-        if (lane == 0) {
-            s_redi[wave][0] = ws;
-            s_redi[wave][1] = wh;
-            s_red[wave][0] = w1;
-            s_red[wave][1] = w2;
-            s_red[wave][2] = w3;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            TilePartial tp;
-            tp.n_steps = 0;
-            tp.n_hits = 0;
-            tp.sum_w = 0.0;
-            tp.sum_wt = 0.0;
-            tp.sum_wt2 = 0.0;
-            for (int q = 0; q < kWaves; ++q) {
-                tp.n_steps += s_redi[q][0];
-                tp.n_hits += s_redi[q][1];
-                tp.sum_w += s_red[q][0];
-                tp.sum_wt += s_red[q][1];
-                tp.sum_wt2 += s_red[q][2];
-            }
-            P.partials[tile] = tp;
-        }
+        const TilePartial tp = run_tile(T, R, r, first, count, P.hits, P.hits_cap, P.hit_cursor,
+                                        s_red, s_redi);
+        if (threadIdx.x == 0) P.partials[tile] = tp;
     }
 }
 
-// Per-request sums over its tiles in tile order (one wave per request).
-__global__ void __launch_bounds__(64) reduce_kernel(const TilePartial* partials,
-                                                    const uint64_t* tile_start,
-                                                    const DevReq* reqs, uint32_t n_req,
-                                                    pp_summary* out) {
-    const uint32_t r = blockIdx.x;
-    if (r >= n_req) return;
-    const uint32_t lane = threadIdx.x;
-    const uint64_t b = tile_start[r], e = tile_start[r + 1];
+// Sum of a request's tile partials in tile order: lane l adds tiles l, l+64, ...
+// then a fixed butterfly.  Valid in every lane of the calling wave.
+__device__ __forceinline__ pp_summary reduce_partials(const TilePartial* parts, uint64_t n_tiles,
+                                                      uint64_t n_sim) {
+    const uint32_t lane = threadIdx.x & 63u;
     uint64_t st = 0, nh = 0;
     double sw = 0.0, swt = 0.0, swt2 = 0.0;
-    for (uint64_t t = b + lane; t < e; t += 64) {
-        const TilePartial p = partials[t];
+    for (uint64_t t = lane; t < n_tiles; t += 64) {
+        const TilePartial p = parts[t];
         st += p.n_steps;
         nh += p.n_hits;
         sw += p.sum_w;
@@ -350,19 +388,29 @@ __global__ void __launch_bounds__(64) reduce_kernel(const TilePartial* partials,
         swt += __shfl_xor(swt, o, 64);
         swt2 += __shfl_xor(swt2, o, 64);
     }
-    if (lane == 0) {
-        pp_summary s;
-        s.n_sim = reqs[r].n_photons;
-        s.n_steps = st;
-        s.n_hits = nh;
-        s.n_stored = 0;
-        s.sum_w = sw;
-        s.sum_wt = swt;
-        s.sum_wt2 = swt2;
-        s.status = PP_OK;
-        s.reserved = 0;
-        out[r] = s;
-    }
+    pp_summary s;
+    s.n_sim = n_sim;
+    s.n_steps = st;
+    s.n_hits = nh;
+    s.n_stored = 0;
+    s.sum_w = sw;
+    s.sum_wt = swt;
+    s.sum_wt2 = swt2;
+    s.status = PP_OK;
+    s.reserved = 0;
+    return s;
+}
+
+// Per-request sums over its tiles in tile order (one wave per request).
+__global__ void __launch_bounds__(64) reduce_kernel(const TilePartial* partials,
+                                                    const uint64_t* tile_start,
+                                                    const DevReq* reqs, uint32_t n_req,
+                                                    pp_summary* out) {
+    const uint32_t r = blockIdx.x;
+    if (r >= n_req) return;
+    const uint64_t b = tile_start[r], e = tile_start[r + 1];
+    const pp_summary s = reduce_partials(partials + b, e - b, reqs[r].n_photons);
+    if (threadIdx.x == 0) out[r] = s;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
@@ -371,13 +419,39 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-// Stored hits per request.
-__global__ void count_kernel(const pp_hit* staged, const unsigned long long* cursor,
-                             uint64_t cap, uint32_t* cnt) {
+// Hit compaction.  Staged records [0, min(cursor, cap)) are split into
+// gridDim.x contiguous chunks, the same in both passes.  Per chunk, an LDS
+// histogram counts records per request, so a block issues one global atomic
+// per request it saw instead of one per record (the staged order follows the
+// tiles, so a chunk mostly holds one or two requests).  Requests beyond
+// kLdsHistMax fall back to per-record global atomics.
+constexpr uint32_t kLdsHistMax = 4096;
+
+__device__ __forceinline__ uint32_t staged_request(const pp_hit* staged, uint64_t i) {
+    return reinterpret_cast<const uint32_t*>(staged + i)[13];
+}
+
+__global__ void __launch_bounds__(256) count_kernel(const pp_hit* staged,
+                                                    const unsigned long long* cursor,
+                                                    uint64_t cap, uint32_t n_req, uint32_t* cnt) {
+    __shared__ uint32_t hist[kLdsHistMax];
     const uint64_t n = *cursor < cap ? *cursor : cap;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[staged[i].request], 1u);
+    const bool lds = n_req <= kLdsHistMax;
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = blockIdx.x * chunk;
+    const uint64_t e = b + chunk < n ? b + chunk : n;
+    if (lds)
+        for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x) hist[r] = 0;
+    __syncthreads();
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t r = staged_request(staged, i);
+        if (lds) atomicAdd(&hist[r], 1u);
+        else atomicAdd(&cnt[r], 1u);
+    }
+    __syncthreads();
+    if (lds)
+        for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x)
+            if (hist[r]) atomicAdd(&cnt[r], hist[r]);
 }
 
 // Exclusive scan of cnt into offsets[0..n], single block of 1024 threads;
@@ -409,19 +483,40 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t* cnt, uint32_
     if (t == 1023) offsets[n] = s_part[1023];
 }
 
-__global__ void scatter_kernel(const pp_hit* staged, const unsigned long long* cursor,
-                               uint64_t cap, const uint64_t* offsets, uint32_t* fill,
-                               pp_hit* out) {
+__global__ void __launch_bounds__(256) scatter_kernel(const pp_hit* staged,
+                                                      const unsigned long long* cursor,
+                                                      uint64_t cap, uint32_t n_req,
+                                                      const uint64_t* offsets, uint32_t* fill,
+                                                      pp_hit* out) {
+    __shared__ uint32_t hist[kLdsHistMax];
+    __shared__ uint64_t base[kLdsHistMax];
     const uint64_t n = *cursor < cap ? *cursor : cap;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
+    const bool lds = n_req <= kLdsHistMax;
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = blockIdx.x * chunk;
+    const uint64_t e = b + chunk < n ? b + chunk : n;
+    if (lds) {
+        for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x) hist[r] = 0;
+        __syncthreads();
+        for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x)
+            atomicAdd(&hist[staged_request(staged, i)], 1u);
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x) {
+            const uint32_t h = hist[r];
+            base[r] = h ? offsets[r] + atomicAdd(&fill[r], h) : 0;
+            hist[r] = 0;
+        }
+        __syncthreads();
+    }
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const uint4* src = reinterpret_cast<const uint4*>(staged + i);
-        const uint4 a = src[0], b = src[1], c = src[2], d = src[3];
+        const uint4 a = src[0], bb = src[1], c = src[2], d = src[3];
         const uint32_t r = d.y;
-        const uint64_t slot = offsets[r] + atomicAdd(&fill[r], 1u);
+        const uint64_t slot = lds ? base[r] + atomicAdd(&hist[r], 1u)
+                                  : offsets[r] + atomicAdd(&fill[r], 1u);
         uint4* dst = reinterpret_cast<uint4*>(out + slot);
         dst[0] = a;
-        dst[1] = b;
+        dst[1] = bb;
         dst[2] = c;
         dst[3] = d;
     }
@@ -685,6 +780,17 @@ int check_cone(const pp_ctx* c, const pp_request* reqs, uint32_t n_req, float n_
 }
 
 
+// Henyey-Greenstein constants for the device: 1-g², 1+g², 1/(2g).
+ppd::HG make_hg(float g) {
+    ppd::HG h;
+    h.g = g;
+    h.iso = g == 0.0f;
+    h.num = std::fma(-g, g, 1.0f);
+    h.opg2 = std::fma(g, g, 1.0f);
+    h.inv2g = h.iso ? 0.0f : 1.0f / (2.0f * g);
+    return h;
+}
+
 size_t dyn_lds_bytes(const pp_ctx* c, const SpecParams& sp) {
     return (size_t)c->n_points * sizeof(float4) +
            (sp.kind == PP_SPEC_KNOTS ? (size_t)sp.n_knots * sizeof(float) : 0);
@@ -750,11 +856,7 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
     P.knots = (const float*)c->knots.p;
     P.lmin = c->lmin;
     P.inv_dl = c->inv_dl;
-    P.hg.g = c->g;
-    P.hg.iso = c->g == 0.0f;
-    P.hg.num = std::fma(-c->g, c->g, 1.0f);
-    P.hg.opg2 = std::fma(c->g, c->g, 1.0f);
-    P.hg.inv2g = P.hg.iso ? 0.0f : 1.0f / (2.0f * c->g);
+    P.hg = make_hg(c->g);
     P.spec = sp;
     P.hits = d_staging;
     P.hits_cap = hits_cap;
@@ -786,10 +888,11 @@ int enqueue_compaction(pp_ctx* c, uint32_t n_req, const pp_hit* d_staging, uint6
     fill_u32_kernel<<<zb, 256, 0, c->stream>>>((uint32_t*)c->fill.p, n_req, 0u);
     const unsigned long long* cursor = (const unsigned long long*)((uint32_t*)c->counters.p + 2);
     const int gb = 2048;
-    count_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, (uint32_t*)c->cnt.p);
+    count_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req,
+                                            (uint32_t*)c->cnt.p);
     scan_kernel<<<1, 1024, 0, c->stream>>>((const uint32_t*)c->cnt.p, n_req, d_offsets,
                                            (pp_summary*)c->sums.p);
-    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, d_offsets,
+    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req, d_offsets,
                                               (uint32_t*)c->fill.p, d_out);
     PP_HIP(hipGetLastError());
     return PP_OK;
@@ -1010,16 +1113,6 @@ int pp_selftest_philox(pp_ctx* c, const uint32_t* ctr, uint64_t seed, uint32_t* 
     return PP_OK;
 }
 
-static ppd::HG make_hg(float g) {
-    ppd::HG h;
-    h.g = g;
-    h.iso = g == 0.0f;
-    h.num = std::fma(-g, g, 1.0f);
-    h.opg2 = std::fma(g, g, 1.0f);
-    h.inv2g = h.iso ? 0.0f : 1.0f / (2.0f * g);
-    return h;
-}
-
 int pp_selftest_hg(pp_ctx* c, float g, uint64_t seed, float* out, uint64_t n) {
     int rc = validate_ctx(c);
     if (rc) return rc;
@@ -1056,6 +1149,635 @@ int pp_selftest_hg_moments(pp_ctx* c, float g, uint64_t seed, uint64_t n, double
     }
     sums_out[0] = a;
     sums_out[1] = b;
+    return PP_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Serving loop: continuous batching into a persistent kernel
+// ===========================================================================
+//
+// Host side (worker thread): pending FIFO -> admission when a slot and
+// hits_cap records of the device hit pool are free -> the request's slot
+// descriptor and a range entry [t0, t1) of the global tile sequence are
+// written to pinned host memory -> `published` advances past t1.
+// Device side (persistent kernel, one resident block per CU slot): a block
+// claims the next global tile with a CAS on a device counter (only while the
+// tile is below `published`), finds its range entry by a 64-ary search, runs
+// the tile with the same run_tile as the batch kernel, stores the tile
+// partial, and the block that completes a request's last tile adds the
+// partials in tile order and posts the summary to pinned memory.  Blocks
+// leave when the host sets `stop` or after idle_exit_us without work.
+
+namespace {
+
+constexpr uint32_t kSrvRing = 16384;          // range-entry ring
+constexpr uint64_t kSrvExit = ~0ull;
+
+struct SrvEntry {
+    uint64_t t0, t1;
+    uint32_t slot, pad;
+    uint64_t pad2;
+};
+
+struct SrvSlot {       // written by the host before the tiles are published
+    DevReq req;
+    uint64_t n_tiles;
+    uint64_t hit_base;  // record offset into the hit pool
+    uint64_t hit_cap;
+    uint64_t seq;       // completion tag
+};
+
+struct SrvDone {       // written by the device
+    uint64_t words[8];  // pp_summary
+    uint64_t seq;
+    uint64_t pad[7];
+};
+
+struct SrvControl {
+    uint64_t published;   // end of the published global tile range
+    uint64_t n_entries;   // range entries published
+    uint64_t first_live;  // oldest entry that can still be searched
+    uint64_t stop;
+    uint64_t pad[4];
+};
+
+struct SrvParams {
+    SrvControl* ctl;
+    const SrvEntry* entries;
+    const SrvSlot* slots;
+    SrvDone* done;
+    unsigned long long* next_tile;
+    uint32_t* tiles_done;             // [max_slots]
+    unsigned long long* hit_cursor;   // [max_slots]
+    TilePartial* partials;            // [max_slots][max_tiles]
+    pp_hit* pool;
+    uint32_t max_tiles;
+    uint32_t n_points;
+    const float4* table;
+    const float* knots;
+    float lmin, inv_dl;
+    ppd::HG hg;
+    SpecParams spec;
+    uint64_t idle_ticks;              // s_memrealtime ticks (100 MHz)
+    uint64_t life_ticks;
+};
+
+template <typename T>
+__device__ __forceinline__ T sys_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <typename T>
+__device__ __forceinline__ T sys_load_acq(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
+    extern __shared__ float4 s_table[];
+    __shared__ unsigned long long s_tile;
+    __shared__ SrvSlot s_desc;
+    __shared__ uint64_t s_entry_t0;
+    __shared__ uint32_t s_slot;
+    __shared__ uint32_t s_last;
+    __shared__ double s_red[kWaves][3];
+    __shared__ uint32_t s_redi[kWaves][2];
+    const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
+                                   P.inv_dl, P.hg);
+    const uint32_t tid = threadIdx.x;
+    const uint64_t born = __builtin_amdgcn_s_memrealtime();
+    uint64_t idle_since = born;
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long got = kSrvExit;
+            for (;;) {
+                const uint64_t pub = sys_load_acq(&P.ctl->published);
+                const unsigned long long t = __hip_atomic_load(
+                    P.next_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t < pub) {
+                    if (atomicCAS(P.next_tile, t, t + 1) == t) {
+                        got = t;
+                        break;
+                    }
+                    continue;
+                }
+                const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                if (sys_load(&P.ctl->stop) || now - idle_since > P.idle_ticks ||
+                    now - born > P.life_ticks)
+                    break;
+                __builtin_amdgcn_s_sleep(64);
+            }
+            s_tile = got;
+        }
+        __syncthreads();
+        const unsigned long long t = s_tile;
+        if (t == kSrvExit) break;
+
+        // Range entry holding tile t: 64-ary search over live entries (wave 0).
+        if (tid < 64) {
+            // one read, broadcast from lane 0, so the search is wave-uniform
+            uint64_t lo = __shfl(sys_load(&P.ctl->first_live), 0, 64);
+            uint64_t hi = __shfl(sys_load(&P.ctl->n_entries), 0, 64);
+            while (hi - lo > 1) {
+                const uint64_t stride = (hi - lo + 63) / 64;
+                const uint64_t idx = lo + tid * stride;
+                const bool ok = idx < hi && sys_load(&P.entries[idx % kSrvRing].t0) <= t;
+                const uint64_t m = __ballot(ok);
+                const uint32_t l = 63u - (uint32_t)__clzll((long long)m);  // lane 0 always ok
+                lo = lo + (uint64_t)l * stride;
+                hi = std::min(hi, lo + stride);
+            }
+            if (tid == 0) {
+                const SrvEntry* e = &P.entries[lo % kSrvRing];
+                s_entry_t0 = sys_load(&e->t0);
+                s_slot = sys_load(&e->slot);
+            }
+        }
+        __syncthreads();
+        const uint32_t slot = s_slot;
+        if (tid < sizeof(SrvSlot) / 4)
+            reinterpret_cast<uint32_t*>(&s_desc)[tid] =
+                sys_load(reinterpret_cast<const uint32_t*>(&P.slots[slot]) + tid);
+        __syncthreads();
+        const SrvSlot* sd = &s_desc;
+        const DevReq R = sd->req;
+        const uint64_t local = t - s_entry_t0;
+        const uint64_t first = local * kTilePhotons;
+        const uint64_t left = R.n_photons - first;
+        const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
+        const uint64_t n_tiles = sd->n_tiles;
+        const uint64_t hit_cap = sd->hit_cap;
+        const uint64_t seq = sd->seq;
+        TilePartial* parts = P.partials + (uint64_t)slot * P.max_tiles;
+        const TilePartial tp = run_tile(T, R, slot, first, count, P.pool + sd->hit_base, hit_cap,
+                                        &P.hit_cursor[slot], s_red, s_redi);
+        __syncthreads();  // every lane's hit stores issued before the fence below
+        if (tid == 0) {
+            parts[local] = tp;
+            __threadfence_system();  // partial and hit records visible to other XCDs and DMA
+            const uint32_t done = atomicAdd(&P.tiles_done[slot], 1u) + 1u;
+            s_last = done == n_tiles;
+            __threadfence();
+        }
+        __syncthreads();
+        if (s_last && tid < 64) {
+            const pp_summary s = reduce_partials(parts, n_tiles, R.n_photons);
+            if (tid == 0) {
+                const unsigned long long cur = __hip_atomic_load(
+                    &P.hit_cursor[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pp_summary o = s;
+                o.n_stored = cur < hit_cap ? cur : hit_cap;
+                o.status = o.n_hits > o.n_stored ? PP_ERR_OVERFLOW : PP_OK;
+                const uint64_t* w = reinterpret_cast<const uint64_t*>(&o);
+                for (int k = 0; k < 8; ++k)
+                    __hip_atomic_store(&P.done[slot].words[k], w[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                P.tiles_done[slot] = 0;
+                P.hit_cursor[slot] = 0;
+                __threadfence_system();
+                __hip_atomic_store(&P.done[slot].seq, seq, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        idle_since = __builtin_amdgcn_s_memrealtime();
+    }
+}
+
+struct Job {
+    uint64_t ticket = 0;
+    pp_request req;
+    uint64_t hits_cap = 0;
+    uint64_t n_tiles = 0;
+    uint32_t slot = 0;
+    uint64_t hit_base = 0;
+    uint64_t seq = 0;
+    bool done = false;
+    pp_summary summary;
+    std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+}  // namespace
+
+struct pp_server {
+    pp_ctx* ctx = nullptr;
+    pp_server_config cfg;
+    SpecParams spec;
+    size_t lds = 0;
+    int grid = 0;
+    uint32_t n_points = 0;
+    float lmin = 0.0f, inv_dl = 0.0f, g = 0.0f;
+    hipStream_t kstream = nullptr, cstream = nullptr;
+    hipEvent_t kdone = nullptr;
+    bool kernel_live = false;
+    uint64_t launches = 0;
+    // pinned
+    SrvControl* ctl = nullptr;
+    SrvEntry* entries = nullptr;
+    SrvSlot* slots = nullptr;
+    SrvDone* done = nullptr;
+    // device
+    DevBuf table, knots, next_tile, tiles_done, hit_cursor, partials, pool;
+    // host bookkeeping (guarded by mu)
+    std::mutex mu;
+    std::condition_variable cv;
+    std::thread worker;
+    bool shutdown = false;
+    std::deque<uint64_t> pending;
+    std::map<uint64_t, Job> jobs;
+    std::vector<uint32_t> free_slots;
+    std::vector<uint64_t> slot_ticket;
+    std::vector<uint64_t> entry_ticket;       // ring: ticket of each live entry
+    std::map<uint64_t, uint64_t> pool_free;   // offset -> length (records)
+    uint64_t pool_free_total = 0;
+    uint64_t next_ticket = 1, next_seq = 1;
+    uint64_t published = 0, n_entries = 0, first_live = 0;
+    uint64_t submitted = 0, completed = 0;
+    std::string worker_error;
+};
+
+namespace {
+
+bool pool_alloc(pp_server* s, uint64_t n, uint64_t* off) {
+    if (n == 0) {
+        *off = 0;
+        return true;
+    }
+    for (auto it = s->pool_free.begin(); it != s->pool_free.end(); ++it) {
+        if (it->second >= n) {
+            *off = it->first;
+            const uint64_t rest = it->second - n, at = it->first + n;
+            s->pool_free.erase(it);
+            if (rest) s->pool_free[at] = rest;
+            s->pool_free_total -= n;
+            return true;
+        }
+    }
+    return false;
+}
+
+void pool_release(pp_server* s, uint64_t off, uint64_t n) {
+    if (n == 0) return;
+    s->pool_free_total += n;
+    auto it = s->pool_free.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != s->pool_free.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        s->pool_free.erase(nx);
+    }
+    if (it != s->pool_free.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            s->pool_free.erase(it);
+        }
+    }
+}
+
+int launch_serve_kernel(pp_server* s) {
+    SrvParams P;
+    memset(&P, 0, sizeof P);
+    P.ctl = s->ctl;
+    P.entries = s->entries;
+    P.slots = s->slots;
+    P.done = s->done;
+    P.next_tile = (unsigned long long*)s->next_tile.p;
+    P.tiles_done = (uint32_t*)s->tiles_done.p;
+    P.hit_cursor = (unsigned long long*)s->hit_cursor.p;
+    P.partials = (TilePartial*)s->partials.p;
+    P.pool = (pp_hit*)s->pool.p;
+    P.max_tiles = s->cfg.max_tiles_per_request;
+    P.n_points = s->n_points;
+    P.table = (const float4*)s->table.p;
+    P.knots = (const float*)s->knots.p;
+    P.lmin = s->lmin;
+    P.inv_dl = s->inv_dl;
+    P.hg = make_hg(s->g);
+    P.spec = s->spec;
+    P.idle_ticks = (uint64_t)s->cfg.idle_exit_us * 100;  // 100 MHz
+    P.life_ticks = 30ull * 100000000ull;                  // relaunched by the worker
+    serve_kernel<<<s->grid, kBlock, s->lds, s->kstream>>>(P);
+    PP_HIP(hipGetLastError());
+    PP_HIP(hipEventRecord(s->kdone, s->kstream));
+    s->kernel_live = true;
+    ++s->launches;
+    return PP_OK;
+}
+
+// Admit pending jobs (FIFO) while a slot, pool room and ring room exist.
+void admit_locked(pp_server* s) {
+    bool published_any = false;
+    while (!s->pending.empty() && !s->free_slots.empty() &&
+           s->n_entries - s->first_live < kSrvRing) {
+        Job& j = s->jobs[s->pending.front()];
+        uint64_t off = 0;
+        if (!pool_alloc(s, j.hits_cap, &off)) break;
+        s->pending.pop_front();
+        j.slot = s->free_slots.back();
+        s->free_slots.pop_back();
+        j.hit_base = off;
+        j.seq = s->next_seq++;
+        s->slot_ticket[j.slot] = j.ticket;
+        if (j.n_tiles == 0) {  // nothing to run: complete at once
+            memset(&j.summary, 0, sizeof j.summary);
+            j.done = true;
+            j.t_done = std::chrono::steady_clock::now();
+            s->free_slots.push_back(j.slot);
+            ++s->completed;
+            continue;
+        }
+        SrvSlot sd;
+        memset(&sd, 0, sizeof sd);
+        make_devreq(&j.req, j.slot, &sd.req);
+        sd.n_tiles = j.n_tiles;
+        sd.hit_base = off;
+        sd.hit_cap = j.hits_cap;
+        sd.seq = j.seq;
+        memcpy(&s->slots[j.slot], &sd, sizeof sd);
+        const uint64_t e = s->n_entries;
+        SrvEntry en;
+        memset(&en, 0, sizeof en);
+        en.t0 = s->published;
+        en.t1 = s->published + j.n_tiles;
+        en.slot = j.slot;
+        memcpy(&s->entries[e % kSrvRing], &en, sizeof en);
+        s->entry_ticket[e % kSrvRing] = j.ticket;
+        s->published = en.t1;
+        s->n_entries = e + 1;
+        published_any = true;
+    }
+    if (published_any) {
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+        __atomic_store_n(&s->ctl->n_entries, s->n_entries, __ATOMIC_RELEASE);
+        __atomic_store_n(&s->ctl->published, s->published, __ATOMIC_RELEASE);
+    }
+}
+
+// Completions posted by the device; advance first_live past finished entries.
+bool reap_locked(pp_server* s) {
+    bool any = false;
+    for (uint32_t slot = 0; slot < s->cfg.max_slots; ++slot) {
+        const uint64_t tk = s->slot_ticket[slot];
+        if (!tk) continue;
+        Job& j = s->jobs[tk];
+        if (j.done) continue;
+        if (__atomic_load_n(&s->done[slot].seq, __ATOMIC_ACQUIRE) != j.seq) continue;
+        memcpy(&j.summary, s->done[slot].words, sizeof(pp_summary));
+        j.done = true;
+        j.t_done = std::chrono::steady_clock::now();
+        s->slot_ticket[slot] = 0;
+        s->free_slots.push_back(slot);
+        ++s->completed;
+        any = true;
+    }
+    while (s->first_live < s->n_entries) {
+        const uint64_t tk = s->entry_ticket[s->first_live % kSrvRing];
+        auto it = s->jobs.find(tk);
+        if (it != s->jobs.end() && !it->second.done) break;
+        ++s->first_live;
+    }
+    if (any) __atomic_store_n(&s->ctl->first_live, s->first_live, __ATOMIC_RELEASE);
+    return any;
+}
+
+void worker_main(pp_server* s) {
+    if (hipSetDevice(s->ctx->device) != hipSuccess) {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->worker_error = "hipSetDevice failed in the serving worker";
+        return;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->shutdown) {
+        const bool any = reap_locked(s);
+        admit_locked(s);
+        if (any) s->cv.notify_all();
+        const bool busy = s->completed < s->submitted;
+        if (s->kernel_live && hipEventQuery(s->kdone) == hipSuccess) s->kernel_live = false;
+        if (busy && !s->kernel_live && s->published > 0) {
+            if (launch_serve_kernel(s) != PP_OK) {
+                s->worker_error = g_err;
+                s->cv.notify_all();
+                break;
+            }
+        }
+        if (busy) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        } else {
+            s->cv.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+void free_server(pp_server* s) {
+    if (s->ctl) (void)hipHostFree(s->ctl);
+    if (s->entries) (void)hipHostFree(s->entries);
+    if (s->slots) (void)hipHostFree(s->slots);
+    if (s->done) (void)hipHostFree(s->done);
+    DevBuf* bufs[] = {&s->table, &s->knots, &s->next_tile, &s->tiles_done, &s->hit_cursor,
+                      &s->partials, &s->pool};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (s->kdone) (void)hipEventDestroy(s->kdone);
+    if (s->kstream) (void)hipStreamDestroy(s->kstream);
+    if (s->cstream) (void)hipStreamDestroy(s->cstream);
+    delete s;
+}
+
+int server_setup(pp_server* s, const pp_spectrum* spectrum) {
+    pp_ctx* c = s->ctx;
+    int rc;
+    PP_HIP(hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking));
+    PP_HIP(hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking));
+    PP_HIP(hipEventCreateWithFlags(&s->kdone, hipEventDisableTiming));
+    // The spectrum's knots go to the context's knot buffer first; keep a copy.
+    if ((rc = prepare_spectrum(c, spectrum, &s->spec))) return rc;
+    if ((rc = ensure(s->knots, PP_MAX_KNOTS * sizeof(float)))) return rc;
+    // the server keeps its own copy of the medium table (pp_set_medium on the
+    // context does not change requests already being served)
+    if ((rc = ensure(s->table, (size_t)c->n_points * sizeof(float4)))) return rc;
+    PP_HIP(hipMemcpyAsync(s->table.p, c->table.p, (size_t)c->n_points * sizeof(float4),
+                          hipMemcpyDeviceToDevice, c->stream));
+    s->n_points = c->n_points;
+    s->lmin = c->lmin;
+    s->inv_dl = c->inv_dl;
+    s->g = c->g;
+    if (s->spec.kind == PP_SPEC_KNOTS)
+        PP_HIP(hipMemcpyAsync(s->knots.p, c->knots.p, s->spec.n_knots * sizeof(float),
+                              hipMemcpyDeviceToDevice, c->stream));
+    PP_HIP(hipStreamSynchronize(c->stream));
+    const uint32_t S = s->cfg.max_slots;
+    PP_HIP(hipHostMalloc((void**)&s->ctl, sizeof(SrvControl), hipHostMallocCoherent));
+    PP_HIP(hipHostMalloc((void**)&s->entries, kSrvRing * sizeof(SrvEntry), hipHostMallocCoherent));
+    PP_HIP(hipHostMalloc((void**)&s->slots, S * sizeof(SrvSlot), hipHostMallocCoherent));
+    PP_HIP(hipHostMalloc((void**)&s->done, S * sizeof(SrvDone), hipHostMallocCoherent));
+    memset(s->ctl, 0, sizeof(SrvControl));
+    memset(s->entries, 0, kSrvRing * sizeof(SrvEntry));
+    memset(s->slots, 0, S * sizeof(SrvSlot));
+    memset(s->done, 0, S * sizeof(SrvDone));
+    if ((rc = ensure(s->next_tile, 64))) return rc;
+    if ((rc = ensure(s->tiles_done, S * sizeof(uint32_t)))) return rc;
+    if ((rc = ensure(s->hit_cursor, S * sizeof(unsigned long long)))) return rc;
+    if ((rc = ensure(s->partials, (size_t)S * s->cfg.max_tiles_per_request * sizeof(TilePartial))))
+        return rc;
+    if ((rc = ensure(s->pool, std::max<uint64_t>(s->cfg.hit_pool_records, 1) * sizeof(pp_hit))))
+        return rc;
+    fill_u32_kernel<<<1, 64, 0, s->kstream>>>((uint32_t*)s->next_tile.p, 16, 0u);
+    fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->tiles_done.p, S, 0u);
+    fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->hit_cursor.p, 2ull * S, 0u);
+    PP_HIP(hipGetLastError());
+    PP_HIP(hipStreamSynchronize(s->kstream));
+    s->lds = (size_t)s->n_points * sizeof(float4) +
+             (s->spec.kind == PP_SPEC_KNOTS ? (size_t)s->spec.n_knots * sizeof(float) : 0);
+    int per_cu = 0;
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, s->lds));
+    s->grid = std::max(1, per_cu) * c->n_cu;
+    s->free_slots.reserve(S);
+    for (uint32_t i = 0; i < S; ++i) s->free_slots.push_back(S - 1 - i);
+    s->slot_ticket.assign(S, 0);
+    s->entry_ticket.assign(kSrvRing, 0);
+    s->pool_free_total = s->cfg.hit_pool_records;
+    if (s->cfg.hit_pool_records) s->pool_free[0] = s->cfg.hit_pool_records;
+    return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_server_create(pp_ctx* c, const pp_spectrum* spectrum, const pp_server_config* cfg,
+                     pp_server** out) {
+    int rc = validate_ctx(c);
+    if (rc) return rc;
+    if (!cfg || !out) return fail(PP_ERR_INVALID_ARG, "cfg/out is NULL");
+    *out = nullptr;
+    if (cfg->max_slots < 1 || cfg->max_slots > 4096)
+        return fail(PP_ERR_INVALID_ARG, "max_slots %u not in [1, 4096]", cfg->max_slots);
+    if (cfg->max_tiles_per_request < 1 || cfg->max_tiles_per_request > (1u << 20))
+        return fail(PP_ERR_INVALID_ARG, "max_tiles_per_request %u not in [1, 2^20]",
+                    cfg->max_tiles_per_request);
+    pp_server* s = new pp_server();
+    s->ctx = c;
+    s->cfg = *cfg;
+    if (!s->cfg.idle_exit_us) s->cfg.idle_exit_us = 20000;
+    s->cfg.idle_exit_us = std::min<uint32_t>(s->cfg.idle_exit_us, 10000000u);
+    if ((rc = server_setup(s, spectrum))) {
+        std::string msg = g_err;
+        free_server(s);
+        g_err = msg;
+        return rc;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return PP_OK;
+}
+
+int pp_server_submit(pp_server* s, const pp_request* request, uint64_t hits_cap,
+                     uint64_t* ticket_out) {
+    if (!s || !request || !ticket_out) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    DevReq dr;
+    int rc = make_devreq(request, 0, &dr);
+    if (rc) return rc;
+    if ((rc = check_cone(s->ctx, request, 1, s->ctx->n_min))) return rc;
+    const uint64_t n_tiles = (request->emitter.n_photons + kTilePhotons - 1) / kTilePhotons;
+    if (n_tiles > s->cfg.max_tiles_per_request)
+        return fail(PP_ERR_INVALID_ARG, "request of %llu photons exceeds %u tiles of %d",
+                    (unsigned long long)request->emitter.n_photons, s->cfg.max_tiles_per_request,
+                    PP_TILE_PHOTONS);
+    if (hits_cap > s->cfg.hit_pool_records)
+        return fail(PP_ERR_INVALID_ARG, "hits_cap %llu exceeds the hit pool (%llu records)",
+                    (unsigned long long)hits_cap, (unsigned long long)s->cfg.hit_pool_records);
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->worker_error.empty()) return fail(PP_ERR_DEVICE, "%s", s->worker_error.c_str());
+    Job j;
+    j.ticket = s->next_ticket++;
+    j.req = *request;
+    j.hits_cap = hits_cap;
+    j.n_tiles = n_tiles;
+    j.t_submit = std::chrono::steady_clock::now();
+    s->jobs[j.ticket] = j;
+    s->pending.push_back(j.ticket);
+    ++s->submitted;
+    admit_locked(s);
+    s->cv.notify_all();
+    *ticket_out = j.ticket;
+    return PP_OK;
+}
+
+int pp_server_wait(pp_server* s, uint64_t ticket, uint32_t timeout_ms, int* done_out) {
+    if (!s || !done_out) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return fail(PP_ERR_NOT_FOUND, "unknown ticket %llu", (unsigned long long)ticket);
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
+    while (!it->second.done && s->worker_error.empty()) {
+        if (s->cv.wait_until(lk, deadline) == std::cv_status::timeout) break;
+        it = s->jobs.find(ticket);
+    }
+    if (!s->worker_error.empty()) return fail(PP_ERR_DEVICE, "%s", s->worker_error.c_str());
+    *done_out = it->second.done ? 1 : 0;
+    return PP_OK;
+}
+
+int pp_server_result(pp_server* s, uint64_t ticket, pp_summary* summary, pp_hit* hits_out,
+                     uint64_t hits_cap, uint64_t* n_hits_out, double* latency_us) {
+    if (!s) return fail(PP_ERR_INVALID_ARG, "server is NULL");
+    if (hits_cap && !hits_out) return fail(PP_ERR_INVALID_ARG, "hits_out is NULL with hits_cap > 0");
+    Job j;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->jobs.find(ticket);
+        if (it == s->jobs.end()) return fail(PP_ERR_NOT_FOUND, "unknown ticket %llu", (unsigned long long)ticket);
+        if (!it->second.done) return fail(PP_ERR_BUSY, "ticket %llu not finished", (unsigned long long)ticket);
+        j = it->second;
+    }
+    const uint64_t n = std::min<uint64_t>(j.summary.n_stored, hits_cap);
+    if (n) {
+        PP_HIP(hipSetDevice(s->ctx->device));
+        PP_HIP(hipMemcpyAsync(hits_out, (pp_hit*)s->pool.p + j.hit_base, n * sizeof(pp_hit),
+                              hipMemcpyDeviceToHost, s->cstream));
+        PP_HIP(hipStreamSynchronize(s->cstream));
+    }
+    if (summary) *summary = j.summary;
+    if (n_hits_out) *n_hits_out = n;
+    if (latency_us)
+        *latency_us = std::chrono::duration<double, std::micro>(j.t_done - j.t_submit).count();
+    std::lock_guard<std::mutex> g(s->mu);
+    pool_release(s, j.hit_base, j.hits_cap);
+    s->jobs.erase(ticket);
+    admit_locked(s);
+    s->cv.notify_all();
+    return PP_OK;
+}
+
+int pp_server_stats_get(pp_server* s, pp_server_stats* out) {
+    if (!s || !out) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->pending = s->pending.size();
+    out->in_flight = s->submitted - s->completed - s->pending.size();
+    out->kernel_launches = s->launches;
+    out->pool_records_free = s->pool_free_total;
+    out->tiles_published = s->published;
+    return PP_OK;
+}
+
+int pp_server_destroy(pp_server* s) {
+    if (!s) return PP_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->shutdown = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->ctx->device);
+    if (s->ctl) __atomic_store_n(&s->ctl->stop, 1ull, __ATOMIC_RELEASE);
+    if (s->kstream) (void)hipStreamSynchronize(s->kstream);
+    free_server(s);
     return PP_OK;
 }
 

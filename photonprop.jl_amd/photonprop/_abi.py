@@ -100,6 +100,24 @@ class pp_request(ctypes.Structure):
     ]
 
 
+class pp_server_config(ctypes.Structure):
+    _fields_ = [
+        ("max_slots", ctypes.c_uint32),
+        ("max_tiles_per_request", ctypes.c_uint32),
+        ("hit_pool_records", ctypes.c_uint64),
+        ("idle_exit_us", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class pp_server_stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "completed", "in_flight", "pending", "kernel_launches",
+        "pool_records_free", "tiles_published")]
+
+
+PP_TILE_PHOTONS = 8192
+
 # pp_hit as a numpy record (64 bytes).
 HIT_DTYPE = np.dtype([
     ("pos", "<f4", (3,)),
@@ -141,6 +159,14 @@ SIGNATURES = {
                                                  ctypes.POINTER(pp_spectrum), _vp, _u64, _vp, _vp]),
     "pp_synchronize": (ctypes.c_int, [_vp]),
     "pp_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "pp_server_create": (ctypes.c_int, [_vp, ctypes.POINTER(pp_spectrum), ctypes.POINTER(pp_server_config),
+                                        ctypes.POINTER(_vp)]),
+    "pp_server_submit": (ctypes.c_int, [_vp, ctypes.POINTER(pp_request), _u64, ctypes.POINTER(_u64)]),
+    "pp_server_wait": (ctypes.c_int, [_vp, _u64, _u32, ctypes.POINTER(ctypes.c_int)]),
+    "pp_server_result": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(pp_summary), _vp, _u64,
+                                        ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_double)]),
+    "pp_server_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(pp_server_stats)]),
+    "pp_server_destroy": (ctypes.c_int, [_vp]),
     "pp_selftest_rotate": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64]),
     "pp_selftest_hg": (ctypes.c_int, [_vp, ctypes.c_float, _u64, _vp, _u64]),
     "pp_selftest_philox": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64]),

@@ -1,0 +1,123 @@
+"""Serving loop: continuous batching of propagation requests into a
+persistent GPU kernel (pp_server_* in include/photonprop.h).
+
+Requests arrive with uneven photon budgets; the library admits them as device
+slots and hit-pool room (the HBM budget) free up, and publishes their photon
+tiles to a kernel that keeps running while work arrives.  A request's result
+is bitwise the same as Context.propagate gives for it alone.
+"""
+import ctypes
+import threading
+import time
+
+import numpy as np
+
+from . import _abi
+from .propagation import REQUEST_DTYPE, spectrum_struct, summary_dict
+
+
+class Server:
+    def __init__(self, ctx, spectrum, max_slots=256, max_tiles_per_request=16384,
+                 hit_pool_records=1 << 24, idle_exit_us=20000):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = _abi.pp_server_config(max_slots, max_tiles_per_request, int(hit_pool_records),
+                                    int(idle_exit_us), 0)
+        spec, keep = spectrum_struct(spectrum)
+        self._h = ctypes.c_void_p()
+        _abi.check(self.lib.pp_server_create(ctx.handle, ctypes.byref(spec), ctypes.byref(cfg),
+                                             ctypes.byref(self._h)), self.lib)
+        del keep
+
+    def submit(self, request, hits_cap=0):
+        """Queue one REQUEST_DTYPE record; returns a ticket."""
+        r = np.ascontiguousarray(np.asarray(request, dtype=REQUEST_DTYPE).reshape(1))
+        t = ctypes.c_uint64()
+        _abi.check(self.lib.pp_server_submit(self._h, r.ctypes.data_as(ctypes.POINTER(_abi.pp_request)),
+                                             int(hits_cap), ctypes.byref(t)), self.lib)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=10000):
+        done = ctypes.c_int()
+        _abi.check(self.lib.pp_server_wait(self._h, int(ticket), int(timeout_ms), ctypes.byref(done)),
+                   self.lib)
+        return bool(done.value)
+
+    def result(self, ticket, hits_cap=0):
+        """(summary dict, hits, latency in µs) of a finished ticket; frees it."""
+        hits = np.empty(max(int(hits_cap), 0), dtype=_abi.HIT_DTYPE)
+        s = _abi.pp_summary()
+        n = ctypes.c_uint64()
+        lat = ctypes.c_double()
+        _abi.check(self.lib.pp_server_result(self._h, int(ticket), ctypes.byref(s), _abi.ptr(hits),
+                                             int(hits_cap), ctypes.byref(n), ctypes.byref(lat)),
+                   self.lib)
+        return summary_dict(s), hits[: n.value], lat.value
+
+    def stats(self):
+        st = _abi.pp_server_stats()
+        _abi.check(self.lib.pp_server_stats_get(self._h, ctypes.byref(st)), self.lib)
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self._h:
+            self.lib.pp_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_open_loop(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
+    """Submit requests[i] at arrivals[i] seconds (Poisson schedule) from one
+    thread while another collects results in order.  Returns a dict with
+    per-request latencies (µs, submit to completion), the makespan and the
+    photon / step totals."""
+    n = len(requests)
+    tickets = [None] * n
+    lat = np.zeros(n)
+    steps = np.zeros(n, dtype=np.uint64)
+    submitted = threading.Semaphore(0)
+    err = []
+
+    def collect():
+        try:
+            for i in range(n):
+                submitted.acquire()
+                while not server.wait(tickets[i], 1000):
+                    if time.perf_counter() - t0 > timeout_s:
+                        raise TimeoutError("request %d not finished" % i)
+                s, _, l = server.result(tickets[i], hits_cap)
+                lat[i] = l
+                steps[i] = s["n_steps"]
+        except Exception as e:  # surfaced after join
+            err.append(e)
+
+    th = threading.Thread(target=collect)
+    t0 = time.perf_counter()
+    th.start()
+    for i in range(n):
+        delay = arrivals[i] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets[i] = server.submit(requests[i], hits_cap)
+        submitted.release()
+    th.join()
+    makespan = time.perf_counter() - t0
+    if err:
+        raise err[0]
+    return {
+        "latency_us": lat,
+        "makespan_s": makespan,
+        "photons": int(np.asarray(requests["n_photons"], dtype=np.uint64).sum()),
+        "steps": int(steps.sum()),
+    }

@@ -1,0 +1,117 @@
+"""Serving loop: continuous batching into a persistent kernel.
+
+Each request's result must be the same alone (Context.propagate) and in the
+mix (submitted to the server while others are on the GPU): summaries bitwise,
+hit sets bitwise by photon id.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import photonprop as pp
+from photonprop import workloads
+
+pytestmark = pytest.mark.gpu
+SUM_KEYS = ("n_sim", "n_steps", "n_hits", "sum_w", "sum_wt", "sum_wt2")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def requests(n, seed):
+    reqs = workloads.batched_pairs(n, seed=seed, photons=(1e3, 2e6), distance=(0.8, 5.0))
+    reqs["n_photons"][3] = 0
+    return reqs
+
+
+def alone(ctx, r):
+    src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), float(r["t0"]), int(r["n_photons"]),
+                          pp.InverseSquareSpectrum(300.0, 600.0),
+                          pp.AngularEmissionProfile("CherenkovCone"), float(r["beta"]))
+    tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
+    return ctx.propagate(src, tgt, seed=int(r["seed"]), stream=int(r["stream"]),
+                         max_steps=int(r["max_steps"]))
+
+
+def key(h):
+    h = np.sort(h, order="photon_id").copy()
+    h["request"] = 0
+    return h
+
+
+def test_server_results_equal_alone(ctx):
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(40, seed=21)
+    caps = reqs["n_photons"].astype(np.uint64)
+    say("serving: 40 mixed requests, max_slots 8 (so later ones queue while earlier run)")
+    with pp.Server(ctx, spec, max_slots=8, max_tiles_per_request=512,
+                   hit_pool_records=1 << 24) as srv:
+        tickets = []
+        for i, r in enumerate(reqs):
+            tickets.append(srv.submit(r, int(caps[i])))
+            if i % 5 == 4:
+                time.sleep(0.002)  # arrivals while others are on the GPU
+        got = []
+        for i, t in enumerate(tickets):
+            assert srv.wait(t, 60000), "ticket %d timed out" % i
+            got.append(srv.result(t, int(caps[i])))
+        st = srv.stats()
+    say("server stats", st)
+    assert st["completed"] == st["submitted"] == len(reqs)
+    assert st["kernel_launches"] >= 1
+    for i, r in enumerate(reqs):
+        s, h, lat = got[i]
+        assert lat > 0
+        if i in (0, 1, 2, 3, 7, 11, 19, 39):
+            ref = alone(ctx, r)
+            for k in SUM_KEYS:
+                assert s[k] == ref.summary[k], (i, k, s[k], ref.summary[k])
+            np.testing.assert_array_equal(key(h), key(ref.hits))
+        assert s["n_sim"] == r["n_photons"]
+        assert s["n_stored"] == h.shape[0] == min(s["n_hits"], caps[i])
+
+
+def test_server_overflow_and_summary_only(ctx):
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    r = requests(1, seed=2)[0]
+    r["n_photons"] = 300_000
+    r["center"] = (1.2, 0.0, 0.0)
+    r["axis"] = (1.0, 0.0, 1.0)
+    ref = alone(ctx, r)
+    assert ref.summary["n_hits"] > 10
+    with pp.Server(ctx, spec, max_slots=4, max_tiles_per_request=64, hit_pool_records=1 << 16) as srv:
+        t1 = srv.submit(r, 5)
+        t2 = srv.submit(r, 0)
+        assert srv.wait(t1, 60000) and srv.wait(t2, 60000)
+        s1, h1, _ = srv.result(t1, 5)
+        s2, h2, _ = srv.result(t2, 0)
+        with pytest.raises(pp.PhotonPropError):
+            big = r.copy()
+            big["n_photons"] = 64 * 8192 + 1
+            srv.submit(big, 0)
+        with pytest.raises(pp.PhotonPropError):
+            srv.submit(r, 1 << 17)  # above the pool
+    assert s1["status"] == pp._abi.PP_ERR_OVERFLOW and s1["n_stored"] == 5 and h1.shape[0] == 5
+    assert s2["n_stored"] == 0 and h2.shape[0] == 0
+    for k in SUM_KEYS:
+        assert s1[k] == ref.summary[k] == s2[k]
+
+
+def test_server_open_loop_poisson(ctx):
+    """A short Poisson arrival run of the serving workload shape (budgets
+    log-uniform 1e4..1e7 here to keep the test short); every request finishes
+    and latencies are recorded."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    arr, reqs = workloads.serving_requests(400, 2000.0, seed=5, photons=(1e4, 1e7))
+    with pp.Server(ctx, spec, max_slots=256, max_tiles_per_request=2048,
+                   hit_pool_records=1 << 20) as srv:
+        out = pp.run_open_loop(srv, arr, reqs, hits_cap=0)
+        st = srv.stats()
+    lat = out["latency_us"]
+    say("open loop 2k req/s: p50 %.0f us p99 %.0f us, %.3g photons/s, launches %d"
+        % (np.percentile(lat, 50), np.percentile(lat, 99), out["photons"] / out["makespan_s"],
+           st["kernel_launches"]))
+    assert np.all(lat > 0)
+    assert out["steps"] > 9 * out["photons"]
