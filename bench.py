@@ -205,7 +205,7 @@ def main():
     def step():
         pp._abi.check(lib.pp_propagate_batch_device(
             ctx.handle, creqs, len(reqs), pp._abi.ctypes.byref(spec),
-            pp._abi.ctypes.c_void_p(d_hits.data_ptr()), cap,
+            pp._abi.ctypes.c_void_p(d_hits.data_ptr()), cap, 0,
             pp._abi.ctypes.c_void_p(d_off.data_ptr()), pp._abi.ctypes.c_void_p(d_sum.data_ptr())), lib)
 
     for i in range(args.warmup):

@@ -193,18 +193,25 @@ int pp_propagate(pp_ctx* ctx, const pp_emitter* emitter, const pp_spectrum* spec
  * batching is the unfinished split_source / propagate_sources
  * (src/photon_prop_cuda.jl:474-531).  Hits come back grouped by request:
  * request r owns hits_out[offsets_out[r] .. offsets_out[r+1]).
+ * hits_cap bounds all hits of the launch (exceeding it is PP_ERR_OVERFLOW);
+ * per_request_cap (0: none) keeps at most that many records of each request,
+ * a subset independent of the photons' histories.
  * offsets_out: n_req+1 entries; summaries_out: n_req entries. */
 int pp_propagate_batch(pp_ctx* ctx, const pp_request* requests, uint32_t n_req,
                        const pp_spectrum* spectrum, pp_hit* hits_out, uint64_t hits_cap,
-                       uint64_t* offsets_out, pp_summary* summaries_out);
+                       uint64_t per_request_cap, uint64_t* offsets_out,
+                       pp_summary* summaries_out);
 
 /* Device-resident form of pp_propagate_batch: enqueue on the context's stream and
- * return.  d_hits (hits_cap records), d_offsets (n_req+1) and d_summaries (n_req)
- * are device pointers; results are grouped by request as above and are valid
- * after pp_synchronize(ctx).  Requests are copied before the call returns. */
+ * return.  d_hits, d_offsets (n_req+1) and d_summaries (n_req) are device
+ * pointers; d_hits must hold min(hits_cap, n_req·per_request_cap) records (or
+ * hits_cap when per_request_cap is 0).  Results are grouped by request as above
+ * and are valid after pp_synchronize(ctx).  Requests are copied before the call
+ * returns. */
 int pp_propagate_batch_device(pp_ctx* ctx, const pp_request* requests, uint32_t n_req,
                               const pp_spectrum* spectrum, pp_hit* d_hits, uint64_t hits_cap,
-                              uint64_t* d_offsets, pp_summary* d_summaries);
+                              uint64_t per_request_cap, uint64_t* d_offsets,
+                              pp_summary* d_summaries);
 
 /* Wait for all work enqueued on the context's stream. */
 int pp_synchronize(pp_ctx* ctx);
@@ -266,6 +273,45 @@ int pp_server_stats_get(pp_server* srv, pp_server_stats* out);
 
 /* Drain: stop admitting, let the kernel finish in-flight work and exit. */
 int pp_server_destroy(pp_server* srv);
+
+/* ---- MLP surrogate (BF16 MFMA training and inference) ---------------------- */
+/* Role of PhotonProp.jl's Modelling module (src/modelling.jl:52-111, :202-267):
+ * an MLP n_in -> width^depth -> n_out with SiLU hidden activations, BF16
+ * weights and activations, FP32 accumulation, FP32 master weights and Adam.
+ * The training loss is the Poisson NLL of the observed (weighted) yield k with
+ * expectation N·exp(y0) plus time_weight times the NLL of residual arrival
+ * times (ns) under a 3-component gamma mixture parametrised by y[1..3]
+ * (mixture logits), y[4..6] (log shapes), y[7..9] (log rates per 10 ns).
+ * Parameters are packed layer by layer: Wt_l [n_out_l][n_in_l] then b_l. */
+
+typedef struct pp_mlp pp_mlp;
+
+int pp_mlp_create(pp_ctx* ctx, uint32_t n_in, uint32_t width, uint32_t depth, uint32_t n_out,
+                  uint64_t seed, pp_mlp** out);
+int pp_mlp_destroy(pp_mlp* mlp);
+int pp_mlp_n_params(pp_mlp* mlp, uint64_t* n);
+int pp_mlp_get_params(pp_mlp* mlp, float* params, uint64_t n);
+int pp_mlp_set_params(pp_mlp* mlp, const float* params, uint64_t n);
+
+/* y[n_rows][n_out] = MLP(x[n_rows][n_in]); device pointers, FP32 in and out. */
+int pp_mlp_forward_device(pp_mlp* mlp, const float* d_x, uint64_t n_rows, float* d_y);
+/* Same with host buffers. */
+int pp_mlp_forward(pp_mlp* mlp, const float* x, uint64_t n_rows, float* y);
+/* Duration (ms) of the last pp_mlp_forward_device kernels (HIP events). */
+int pp_mlp_last_forward_ms(pp_mlp* mlp, float* ms);
+
+/* Loss (mean over rows: {Poisson, time}) and its gradient w.r.t. the packed
+ * parameters (grads_host may be NULL).  Device inputs: x [n_rows][n_in],
+ * log_n [n_rows] (log photons), k [n_rows] (observed yield), times
+ * [n_rows][n_times] (residual times in ns; negative = absent). */
+int pp_mlp_loss_grad_device(pp_mlp* mlp, const float* d_x, const float* d_log_n,
+                            const float* d_k, const float* d_times, uint32_t n_times,
+                            uint32_t n_rows, float time_weight, float* grads_host,
+                            double* loss_out);
+/* One Adam step (beta 0.9/0.999) on that loss. */
+int pp_mlp_train_step_device(pp_mlp* mlp, const float* d_x, const float* d_log_n,
+                             const float* d_k, const float* d_times, uint32_t n_times,
+                             uint32_t n_rows, float lr, float time_weight, double* loss_out);
 
 /* ---- device self-tests of the kernel's own helper functions ---------------- */
 /* The reference checks its device helpers through tiny test kernels

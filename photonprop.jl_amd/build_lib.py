@@ -7,7 +7,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", "pp_kernels.hip")]
+SRC = [os.path.join(HERE, "csrc", "pp_kernels.hip"), os.path.join(HERE, "csrc", "pp_mlp.hip")]
 OUT = os.path.join(HERE, "lib", "libphotonprop.so")
 
 HIPCC_FLAGS = [

@@ -1,0 +1,86 @@
+// pp_host.hpp — host-side infrastructure shared by the library's translation
+// units: error reporting, device buffers, the request layout the kernels
+// read, and the context object behind pp_ctx*.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/photonprop.h"
+
+namespace pph {
+
+// Message of the last failure on this thread (pp_last_error).
+std::string& last_error();
+
+// Record a formatted message and return `code`.
+int fail(int code, const char* fmt, ...);
+
+#define PP_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::pph::fail(e_ == hipErrorOutOfMemory ? PP_ERR_OOM : PP_ERR_DEVICE,    \
+                               "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                               __FILE__, __LINE__);                                       \
+    } while (0)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+inline int ensure(DevBuf& b, size_t bytes) {
+    if (bytes <= b.cap) return PP_OK;
+    if (b.p) PP_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    PP_HIP(hipMalloc(&b.p, want));
+    b.cap = want;
+    return PP_OK;
+}
+
+// Request as the kernel reads it (host-precomputed).
+struct DevReq {
+    float pos[3];
+    float t0;
+    float axis[3];
+    float beta;
+    float center[3];
+    float r2;
+    uint64_t n_photons;
+    uint32_t profile;
+    uint32_t max_steps;
+    uint32_t seed_lo, seed_hi, stream, pad;
+};
+
+}  // namespace pph
+
+struct pp_ctx {
+    int device = 0;
+    int n_cu = 0;
+    int blocks_per_cu = 1;
+    char name[256] = {0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    bool timed = false;
+    // medium
+    pph::DevBuf table;
+    uint32_t n_points = 0;
+    float lmin = 0.0f, lmax = 0.0f, inv_dl = 0.0f, g = 0.0f, n_min = 0.0f;
+    pph::DevBuf knots;
+    // work buffers
+    pph::DevBuf reqs, tile_start, partials, counters, staging, cnt, fill, sums, offsets, out_hits;
+    pph::DevBuf scratch;
+    std::vector<pph::DevReq> h_reqs;
+    std::vector<uint64_t> h_tiles;
+};

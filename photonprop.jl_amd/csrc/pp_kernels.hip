@@ -38,6 +38,28 @@
 
 #include "../../include/photonprop.h"
 #include "pp_device.hpp"
+#include "pp_host.hpp"
+
+using namespace pph;
+
+namespace pph {
+
+std::string& last_error() {
+    thread_local std::string msg;
+    return msg;
+}
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    last_error() = buf;
+    return code;
+}
+
+}  // namespace pph
 
 namespace {
 
@@ -50,19 +72,6 @@ constexpr float kLn2 = 0.69314718055994530942f;
 constexpr float kLog2e = 1.44269504088896340736f;
 constexpr float kInv2Pi = 0.15915494309189533577f;
 
-// Request as the kernel reads it (host-precomputed).
-struct DevReq {
-    float pos[3];
-    float t0;
-    float axis[3];
-    float beta;
-    float center[3];
-    float r2;
-    uint64_t n_photons;
-    uint32_t profile;
-    uint32_t max_steps;
-    uint32_t seed_lo, seed_hi, stream, pad;
-};
 
 struct TilePartial {
     uint64_t n_steps;
@@ -456,15 +465,20 @@ __global__ void __launch_bounds__(256) count_kernel(const pp_hit* staged,
 
 // Exclusive scan of cnt into offsets[0..n], single block of 1024 threads;
 // also records n_stored and the overflow status per request.
+__device__ __forceinline__ uint64_t kept(uint32_t c, uint64_t per_cap) {
+    return per_cap && (uint64_t)c > per_cap ? per_cap : (uint64_t)c;
+}
+
 __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t* cnt, uint32_t n,
-                                                    uint64_t* offsets, pp_summary* sums) {
+                                                    uint64_t per_cap, uint64_t* offsets,
+                                                    pp_summary* sums) {
     __shared__ uint64_t s_part[1024];
     const uint32_t t = threadIdx.x;
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t b = t * per;
     const uint32_t e = std::min(n, b + per);
     uint64_t local = 0;
-    for (uint32_t i = b; i < e; ++i) local += cnt[i];
+    for (uint32_t i = b; i < e; ++i) local += kept(cnt[i], per_cap);
     s_part[t] = local;
     __syncthreads();
     for (uint32_t o = 1; o < 1024; o <<= 1) {
@@ -476,8 +490,9 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t* cnt, uint32_
     uint64_t run = s_part[t] - local;
     for (uint32_t i = b; i < e; ++i) {
         offsets[i] = run;
-        run += cnt[i];
-        sums[i].n_stored = cnt[i];
+        run += kept(cnt[i], per_cap);
+        sums[i].n_stored = kept(cnt[i], per_cap);
+        // overflow = records lost to the launch-wide capacity, not to per_cap
         sums[i].status = sums[i].n_hits > (uint64_t)cnt[i] ? PP_ERR_OVERFLOW : PP_OK;
     }
     if (t == 1023) offsets[n] = s_part[1023];
@@ -486,8 +501,8 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t* cnt, uint32_
 __global__ void __launch_bounds__(256) scatter_kernel(const pp_hit* staged,
                                                       const unsigned long long* cursor,
                                                       uint64_t cap, uint32_t n_req,
-                                                      const uint64_t* offsets, uint32_t* fill,
-                                                      pp_hit* out) {
+                                                      uint64_t per_cap, const uint64_t* offsets,
+                                                      uint32_t* fill, pp_hit* out) {
     __shared__ uint32_t hist[kLdsHistMax];
     __shared__ uint64_t base[kLdsHistMax];
     const uint64_t n = *cursor < cap ? *cursor : cap;
@@ -503,7 +518,7 @@ __global__ void __launch_bounds__(256) scatter_kernel(const pp_hit* staged,
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x) {
             const uint32_t h = hist[r];
-            base[r] = h ? offsets[r] + atomicAdd(&fill[r], h) : 0;
+            base[r] = h ? atomicAdd(&fill[r], h) : 0;  // rank of this chunk's first record
             hist[r] = 0;
         }
         __syncthreads();
@@ -512,9 +527,9 @@ __global__ void __launch_bounds__(256) scatter_kernel(const pp_hit* staged,
         const uint4* src = reinterpret_cast<const uint4*>(staged + i);
         const uint4 a = src[0], bb = src[1], c = src[2], d = src[3];
         const uint32_t r = d.y;
-        const uint64_t slot = lds ? base[r] + atomicAdd(&hist[r], 1u)
-                                  : offsets[r] + atomicAdd(&fill[r], 1u);
-        uint4* dst = reinterpret_cast<uint4*>(out + slot);
+        const uint64_t rank = lds ? base[r] + atomicAdd(&hist[r], 1u) : atomicAdd(&fill[r], 1u);
+        if (per_cap && rank >= per_cap) continue;
+        uint4* dst = reinterpret_cast<uint4*>(out + offsets[r] + rank);
         dst[0] = a;
         dst[1] = bb;
         dst[2] = c;
@@ -589,65 +604,8 @@ __global__ void __launch_bounds__(256) selftest_hg_moments_kernel(ppd::HG hg, ui
 // Host side
 // ---------------------------------------------------------------------------
 
-thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define PP_HIP(call)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(e_ == hipErrorOutOfMemory ? PP_ERR_OOM : PP_ERR_DEVICE,           \
-                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,  \
-                        __LINE__);                                                        \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-int ensure(DevBuf& b, size_t bytes) {
-    if (bytes <= b.cap) return PP_OK;
-    if (b.p) PP_HIP(hipFree(b.p));
-    b.p = nullptr;
-    b.cap = 0;
-    size_t want = std::max<size_t>(bytes, 256);
-    PP_HIP(hipMalloc(&b.p, want));
-    b.cap = want;
-    return PP_OK;
-}
-
 }  // namespace
 
-struct pp_ctx {
-    int device = 0;
-    int n_cu = 0;
-    int blocks_per_cu = 1;
-    char name[256] = {0};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_ms = 0.0f;
-    bool timed = false;
-    // medium
-    DevBuf table;
-    uint32_t n_points = 0;
-    float lmin = 0.0f, lmax = 0.0f, inv_dl = 0.0f, g = 0.0f, n_min = 0.0f;
-    DevBuf knots;
-    // work buffers
-    DevBuf reqs, tile_start, partials, counters, staging, cnt, fill, sums, offsets, out_hits;
-    DevBuf scratch;
-    std::vector<DevReq> h_reqs;
-    std::vector<uint64_t> h_tiles;
-};
 
 namespace {
 
@@ -879,7 +837,7 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
 
 // Group staged hits by request into d_out with offsets; fills n_stored/status.
 int enqueue_compaction(pp_ctx* c, uint32_t n_req, const pp_hit* d_staging, uint64_t hits_cap,
-                       pp_hit* d_out, uint64_t* d_offsets) {
+                       uint64_t per_cap, pp_hit* d_out, uint64_t* d_offsets) {
     int rc;
     if ((rc = ensure(c->cnt, n_req * sizeof(uint32_t)))) return rc;
     if ((rc = ensure(c->fill, n_req * sizeof(uint32_t)))) return rc;
@@ -890,10 +848,10 @@ int enqueue_compaction(pp_ctx* c, uint32_t n_req, const pp_hit* d_staging, uint6
     const int gb = 2048;
     count_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req,
                                             (uint32_t*)c->cnt.p);
-    scan_kernel<<<1, 1024, 0, c->stream>>>((const uint32_t*)c->cnt.p, n_req, d_offsets,
+    scan_kernel<<<1, 1024, 0, c->stream>>>((const uint32_t*)c->cnt.p, n_req, per_cap, d_offsets,
                                            (pp_summary*)c->sums.p);
-    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req, d_offsets,
-                                              (uint32_t*)c->fill.p, d_out);
+    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req, per_cap,
+                                              d_offsets, (uint32_t*)c->fill.p, d_out);
     PP_HIP(hipGetLastError());
     return PP_OK;
 }
@@ -905,18 +863,24 @@ int validate_ctx(pp_ctx* c) {
 }
 
 int run_batch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spectrum* spec,
-              pp_hit* d_out, uint64_t hits_cap, uint64_t* d_offsets) {
+              pp_hit* d_out, uint64_t hits_cap, uint64_t per_cap, uint64_t* d_offsets) {
     int rc;
     if ((rc = ensure(c->staging, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
     if ((rc = enqueue_propagation(c, reqs, n_req, spec, (pp_hit*)c->staging.p, hits_cap))) return rc;
-    return enqueue_compaction(c, n_req, (const pp_hit*)c->staging.p, hits_cap, d_out, d_offsets);
+    return enqueue_compaction(c, n_req, (const pp_hit*)c->staging.p, hits_cap, per_cap, d_out,
+                              d_offsets);
+}
+
+// Records the compacted output of a batch can hold.
+uint64_t out_records(uint64_t hits_cap, uint64_t per_cap, uint32_t n_req) {
+    return per_cap ? std::min<uint64_t>(hits_cap, per_cap * n_req) : hits_cap;
 }
 
 }  // namespace
 
 extern "C" {
 
-const char* pp_last_error(void) { return g_err.c_str(); }
+const char* pp_last_error(void) { return last_error().c_str(); }
 
 int pp_abi_version(void) { return PP_ABI_VERSION; }
 
@@ -946,9 +910,9 @@ int pp_create(pp_ctx** out, int device, const pp_medium_tables* medium) {
         return fail(PP_ERR_DEVICE, "stream/event creation: %s", hipGetErrorString(e));
     }
     if ((rc = upload_medium(c, medium))) {
-        std::string msg = g_err;
+        std::string msg = last_error();
         pp_destroy(c);
-        g_err = msg;
+        last_error() = msg;
         return rc;
     }
     *out = c;
@@ -1021,16 +985,18 @@ int pp_propagate(pp_ctx* c, const pp_emitter* emitter, const pp_spectrum* spectr
 
 int pp_propagate_batch(pp_ctx* c, const pp_request* requests, uint32_t n_req,
                        const pp_spectrum* spectrum, pp_hit* hits_out, uint64_t hits_cap,
-                       uint64_t* offsets_out, pp_summary* summaries_out) {
+                       uint64_t per_request_cap, uint64_t* offsets_out,
+                       pp_summary* summaries_out) {
     int rc = validate_ctx(c);
     if (rc) return rc;
     if (hits_cap && !hits_out) return fail(PP_ERR_INVALID_ARG, "hits_out is NULL with hits_cap > 0");
     if (!offsets_out || !summaries_out)
         return fail(PP_ERR_INVALID_ARG, "offsets_out/summaries_out is NULL");
-    if ((rc = ensure(c->out_hits, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
+    const uint64_t out_cap = out_records(hits_cap, per_request_cap, n_req);
+    if ((rc = ensure(c->out_hits, std::max<uint64_t>(out_cap, 1) * sizeof(pp_hit)))) return rc;
     if ((rc = ensure(c->offsets, ((uint64_t)n_req + 1) * sizeof(uint64_t)))) return rc;
     if ((rc = run_batch(c, requests, n_req, spectrum, (pp_hit*)c->out_hits.p, hits_cap,
-                        (uint64_t*)c->offsets.p)))
+                        per_request_cap, (uint64_t*)c->offsets.p)))
         return rc;
     PP_HIP(hipMemcpyAsync(offsets_out, c->offsets.p, ((uint64_t)n_req + 1) * sizeof(uint64_t),
                           hipMemcpyDeviceToHost, c->stream));
@@ -1049,12 +1015,15 @@ int pp_propagate_batch(pp_ctx* c, const pp_request* requests, uint32_t n_req,
 
 int pp_propagate_batch_device(pp_ctx* c, const pp_request* requests, uint32_t n_req,
                               const pp_spectrum* spectrum, pp_hit* d_hits, uint64_t hits_cap,
-                              uint64_t* d_offsets, pp_summary* d_summaries) {
+                              uint64_t per_request_cap, uint64_t* d_offsets,
+                              pp_summary* d_summaries) {
     int rc = validate_ctx(c);
     if (rc) return rc;
     if ((hits_cap && !d_hits) || !d_offsets || !d_summaries)
         return fail(PP_ERR_INVALID_ARG, "device output pointer is NULL");
-    if ((rc = run_batch(c, requests, n_req, spectrum, d_hits, hits_cap, d_offsets))) return rc;
+    if ((rc = run_batch(c, requests, n_req, spectrum, d_hits, hits_cap, per_request_cap,
+                        d_offsets)))
+        return rc;
     PP_HIP(hipMemcpyAsync(d_summaries, c->sums.p, (uint64_t)n_req * sizeof(pp_summary),
                           hipMemcpyDeviceToDevice, c->stream));
     return PP_OK;
@@ -1557,7 +1526,7 @@ void worker_main(pp_server* s) {
         if (s->kernel_live && hipEventQuery(s->kdone) == hipSuccess) s->kernel_live = false;
         if (busy && !s->kernel_live && s->published > 0) {
             if (launch_serve_kernel(s) != PP_OK) {
-                s->worker_error = g_err;
+                s->worker_error = last_error();
                 s->cv.notify_all();
                 break;
             }
@@ -1665,9 +1634,9 @@ int pp_server_create(pp_ctx* c, const pp_spectrum* spectrum, const pp_server_con
     if (!s->cfg.idle_exit_us) s->cfg.idle_exit_us = 20000;
     s->cfg.idle_exit_us = std::min<uint32_t>(s->cfg.idle_exit_us, 10000000u);
     if ((rc = server_setup(s, spectrum))) {
-        std::string msg = g_err;
+        std::string msg = last_error();
         free_server(s);
-        g_err = msg;
+        last_error() = msg;
         return rc;
     }
     s->worker = std::thread(worker_main, s);

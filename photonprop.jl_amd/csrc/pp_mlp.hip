@@ -1,0 +1,714 @@
+// pp_mlp.hip — MLP surrogate of the propagation (training and inference) with
+// hand-written BF16 MFMA kernels for gfx950.
+//
+// Role: PhotonProp.jl's Modelling module fits per-(distance, angle) arrival-time
+// distributions and trains a Flux MLP on them (src/modelling.jl:52-111,
+// :202-267).  Here an MLP maps request features to the log expected yield and a
+// 3-component gamma mixture of the residual arrival time, trained directly on
+// Monte-Carlo output of the propagation kernel.
+//
+// Layout: weights of layer l are Wt_l[n_out][n_in] (row-major, "out x in"),
+// kept in FP32 (master, Adam state) and in two BF16 copies, Wt_l and W_l =
+// Wt_lᵀ.  Every product is one kernel, gemm_nt: C[i][j] = Σ_k A[i][k]·B[j][k]
+// with both operands contiguous along k, which lets each lane fetch its MFMA
+// fragment (8 consecutive bf16 = 16 bytes) straight from global memory:
+//     forward    Z_l  = A_l · Wt_lᵀ              (A = A_l,   B = Wt_l)
+//     backward   dA_l = dZ_l · Wt_l              (A = dZ_l,  B = W_l)
+//                dWt_l = dZ_lᵀ · A_l             (A = dZ_lᵀ, B = A_lᵀ, split over the batch)
+// Inner dimensions are padded to multiples of 16 with zeros.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "pp_host.hpp"
+
+using namespace pph;
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMaxLayers = 16;
+constexpr uint32_t kTimeScale = 10;  // residual times enter the mixture in units of 10 ns
+
+inline uint32_t pad16(uint32_t x) { return (x + 15u) & ~15u; }
+
+enum Epilogue : int {
+    EPI_STORE = 0,      // C = acc (fp32)
+    EPI_ATOMIC = 1,     // C += acc (fp32 atomics; split-K)
+    EPI_BIAS_SILU = 2,  // C = z = acc + bias (fp32), Y = silu(z) (bf16)
+    EPI_SILU_ONLY = 3,  // Y = silu(acc + bias) (bf16), C untouched
+    EPI_BIAS = 4,       // C = acc + bias (fp32)
+};
+
+struct GemmArgs {
+    const bf16* A;
+    const bf16* B;
+    float* C;
+    const float* bias;
+    bf16* Y;
+    uint32_t lda, ldb, ldc, ldy;
+    uint32_t M, N, K, k_split;
+    int epi;
+};
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+// 128x128 block tile, 4 waves of 64x64 (2x2 MFMA 32x32x16 tiles each).
+__global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t i0 = blockIdx.y * 128u + (wave >> 1) * 64u;
+    const uint32_t j0 = blockIdx.x * 128u + (wave & 1u) * 64u;
+    const uint32_t kb = blockIdx.z * g.k_split;
+    const uint32_t ke = min(g.K, kb + g.k_split);
+    const uint32_t r = lane & 31u, kh = (lane >> 5) * 8u;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[a][b][q] = 0.0f;
+    const bf16x8 zero = {};
+    for (uint32_t k = kb; k < ke; k += 16) {
+        bf16x8 fa[2], fb[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const uint32_t row = i0 + t * 32u + r;
+            fa[t] = row < g.M ? *reinterpret_cast<const bf16x8*>(g.A + (size_t)row * g.lda + k + kh)
+                              : zero;
+            const uint32_t col = j0 + t * 32u + r;
+            fb[t] = col < g.N ? *reinterpret_cast<const bf16x8*>(g.B + (size_t)col * g.ldb + k + kh)
+                              : zero;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    // D layout of v_mfma_f32_32x32x16: lane holds column lane%32, rows
+    // 8*(q/4) + 4*(lane/32) + q%4 for q = 0..15.
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const uint32_t j = j0 + b * 32u + r;
+            if (j >= g.N) continue;
+            const float bj = (g.epi >= EPI_BIAS_SILU) ? g.bias[j] : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const uint32_t i = i0 + a * 32u + 8u * (q >> 2) + 4u * (lane >> 5) + (q & 3);
+                if (i >= g.M) continue;
+                const float v = acc[a][b][q];
+                switch (g.epi) {
+                case EPI_STORE: g.C[(size_t)i * g.ldc + j] = v; break;
+                case EPI_ATOMIC: atomicAdd(&g.C[(size_t)i * g.ldc + j], v); break;
+                case EPI_BIAS_SILU: {
+                    const float z = v + bj;
+                    g.C[(size_t)i * g.ldc + j] = z;
+                    g.Y[(size_t)i * g.ldy + j] = (bf16)silu(z);
+                    break;
+                }
+                case EPI_SILU_ONLY: g.Y[(size_t)i * g.ldy + j] = (bf16)silu(v + bj); break;
+                default: g.C[(size_t)i * g.ldc + j] = v + bj; break;
+                }
+            }
+        }
+}
+
+// out[c][r] = in[r][c] for an R x C bf16 matrix (ld_in >= C, ld_out >= R).
+__global__ void __launch_bounds__(256) transpose_kernel(const bf16* in, uint32_t R, uint32_t C,
+                                                        uint32_t ld_in, bf16* out,
+                                                        uint32_t ld_out) {
+    __shared__ bf16 tile[32][33];
+    const uint32_t c0 = blockIdx.x * 32u, r0 = blockIdx.y * 32u;
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;  // 32 x 8
+    for (uint32_t y = ty; y < 32; y += 8) {
+        const uint32_t rr = r0 + y, cc = c0 + tx;
+        tile[y][tx] = (rr < R && cc < C) ? in[(size_t)rr * ld_in + cc] : (bf16)0.0f;
+    }
+    __syncthreads();
+    for (uint32_t y = ty; y < 32; y += 8) {
+        const uint32_t cc = c0 + y, rr = r0 + tx;
+        if (cc < C && rr < R) out[(size_t)cc * ld_out + rr] = tile[tx][y];
+    }
+}
+
+// fp32 [M][n_in] -> bf16 [M][ld] with zero padding (rows beyond M zeroed up to M_pad).
+__global__ void pack_input_kernel(const float* x, uint32_t M, uint32_t M_pad, uint32_t n_in,
+                                  bf16* out, uint32_t ld) {
+    const uint64_t n = (uint64_t)M_pad * ld;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(t / ld), c = (uint32_t)(t % ld);
+        out[t] = (i < M && c < n_in) ? (bf16)x[(size_t)i * n_in + c] : (bf16)0.0f;
+    }
+}
+
+// dZ = dA ⊙ silu'(z), bf16 out (rows >= M zeroed up to M_pad).
+__global__ void silu_backward_kernel(const float* dA, const float* Z, uint32_t M, uint32_t M_pad,
+                                     uint32_t N, uint32_t ld, bf16* dZ) {
+    const uint64_t n = (uint64_t)M_pad * N;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(t / N), j = (uint32_t)(t % N);
+        float d = 0.0f;
+        if (i < M) {
+            const float z = Z[(size_t)i * ld + j];
+            const float s = 1.0f / (1.0f + __expf(-z));
+            d = dA[(size_t)i * ld + j] * s * (1.0f + z * (1.0f - s));
+        }
+        dZ[(size_t)i * ld + j] = (bf16)d;
+    }
+}
+
+// db[j] = Σ_i dZ[i][j]  (one block per 64 columns).
+__global__ void __launch_bounds__(256) bias_grad_kernel(const bf16* dZ, uint32_t M, uint32_t N,
+                                                        uint32_t ld, float* db) {
+    __shared__ float part[4][64];
+    const uint32_t j = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const uint32_t w = threadIdx.x >> 6;
+    float s = 0.0f;
+    if (j < N)
+        for (uint32_t i = w; i < M; i += 4) s += (float)dZ[(size_t)i * ld + j];
+    part[w][threadIdx.x & 63u] = s;
+    __syncthreads();
+    if (w == 0 && j < N) db[j] += part[0][threadIdx.x] + part[1][threadIdx.x] +
+                                  part[2][threadIdx.x] + part[3][threadIdx.x];
+}
+
+__device__ float digamma(float x) {
+    // Recurrence to x >= 6, then the asymptotic series of ψ.
+    float r = 0.0f;
+    while (x < 6.0f) {
+        r -= 1.0f / x;
+        x += 1.0f;
+    }
+    const float f = 1.0f / (x * x);
+    return r + __logf(x) - 0.5f / x -
+           f * (1.0f / 12 - f * (1.0f / 120 - f * (1.0f / 252 - f * (1.0f / 240 - f / 132))));
+}
+
+// Loss per row: Poisson NLL μ - k·log μ with μ = N·exp(y0), plus time_weight ×
+// the mean negative log-likelihood of the row's residual times under the
+// 3-component gamma mixture y[1..3] (logits), y[4..6] (log shape),
+// y[7..9] (log rate).  Gradients (divided by M) go to dY as bf16; loss sums
+// to loss[0] (Poisson) and loss[1] (time) by atomics.
+__global__ void loss_kernel(const float* Y, uint32_t ldy, const float* logn, const float* kobs,
+                            const float* times, uint32_t n_times, uint32_t M, uint32_t M_pad,
+                            float time_weight, bf16* dY, uint32_t ld_dy, double* loss) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M_pad) return;
+    float g[16];
+    for (int c = 0; c < 16; ++c) g[c] = 0.0f;
+    float lp = 0.0f, lt = 0.0f;
+    if (i < M) {
+        const float* y = Y + (size_t)i * ldy;
+        const float inv_m = 1.0f / (float)M;
+        // Poisson
+        const float lmu = fminf(y[0] + logn[i], 60.0f);
+        const float mu = __expf(lmu);
+        lp = mu - kobs[i] * lmu;
+        g[0] = (mu - kobs[i]) * inv_m;
+        // gamma mixture
+        float logit[3], al[3], be[3], lal[3], lbe[3];
+        float mx = fmaxf(y[1], fmaxf(y[2], y[3]));
+        float zs = 0.0f;
+        for (int c = 0; c < 3; ++c) {
+            logit[c] = __expf(y[1 + c] - mx);
+            zs += logit[c];
+            lal[c] = fminf(fmaxf(y[4 + c], -4.0f), 6.0f);
+            lbe[c] = fminf(fmaxf(y[7 + c], -8.0f), 6.0f);
+            al[c] = __expf(lal[c]);
+            be[c] = __expf(lbe[c]);
+        }
+        float pi[3], lg[3], dig[3];
+        for (int c = 0; c < 3; ++c) {
+            pi[c] = logit[c] / zs;
+            lg[c] = lgammaf(al[c]);
+            dig[c] = digamma(al[c]);
+        }
+        uint32_t nt = 0;
+        float gl[3] = {0, 0, 0}, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+        for (uint32_t s = 0; s < n_times; ++s) {
+            const float t = times[(size_t)i * n_times + s];
+            if (!(t >= 0.0f)) continue;
+            const float ts = t / (float)kTimeScale + 1e-3f;
+            const float lt_ = __logf(ts);
+            float lpc[3], m2 = -INFINITY;
+            for (int c = 0; c < 3; ++c) {
+                lpc[c] = __logf(pi[c]) + al[c] * lbe[c] + (al[c] - 1.0f) * lt_ - be[c] * ts - lg[c];
+                m2 = fmaxf(m2, lpc[c]);
+            }
+            float den = 0.0f;
+            for (int c = 0; c < 3; ++c) den += __expf(lpc[c] - m2);
+            lt -= m2 + __logf(den);
+            for (int c = 0; c < 3; ++c) {
+                const float resp = __expf(lpc[c] - m2) / den;
+                gl[c] -= resp - pi[c];
+                ga[c] -= resp * al[c] * (lbe[c] + lt_ - dig[c]);
+                gb[c] -= resp * (al[c] - be[c] * ts);
+            }
+            ++nt;
+        }
+        if (nt) {
+            const float sc = time_weight * inv_m / (float)nt;
+            lt /= (float)nt;
+            for (int c = 0; c < 3; ++c) {
+                g[1 + c] = gl[c] * sc;
+                g[4 + c] = (y[4 + c] > -4.0f && y[4 + c] < 6.0f) ? ga[c] * sc : 0.0f;
+                g[7 + c] = (y[7 + c] > -8.0f && y[7 + c] < 6.0f) ? gb[c] * sc : 0.0f;
+            }
+        }
+    }
+    for (uint32_t c = 0; c < ld_dy; ++c) dY[(size_t)i * ld_dy + c] = (bf16)(c < 16 ? g[c] : 0.0f);
+    if (i < M) {
+        atomicAdd(&loss[0], (double)lp);
+        atomicAdd(&loss[1], (double)lt);
+    }
+}
+
+// Adam (FP32 master weights).
+__global__ void adam_kernel(float* w, const float* g, float* m, float* v, uint64_t n, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const float gi = g[t];
+        const float mi = b1 * m[t] + (1.0f - b1) * gi;
+        const float vi = b2 * v[t] + (1.0f - b2) * gi * gi;
+        m[t] = mi;
+        v[t] = vi;
+        w[t] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// BF16 copies of one layer: Wt [n_out][ldk] and W = Wtᵀ [n_in_pad][ldn] (zero padded).
+__global__ void sync_weights_kernel(const float* wt, uint32_t n_out, uint32_t n_in, bf16* wt16,
+                                    uint32_t ldk, bf16* w16, uint32_t ldn, uint32_t n_in_pad,
+                                    uint32_t n_out_pad) {
+    const uint64_t n = (uint64_t)n_out_pad * ldk;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = (uint32_t)(t / ldk), k = (uint32_t)(t % ldk);
+        const bf16 v = (o < n_out && k < n_in) ? (bf16)wt[(size_t)o * n_in + k] : (bf16)0.0f;
+        if (o < n_out) wt16[(size_t)o * ldk + k] = v;
+        if (k < n_in_pad) w16[(size_t)k * ldn + o] = v;
+    }
+}
+
+// Gradient of layer weights in [n_out][ldk] padded form -> packed [n_out][n_in].
+__global__ void unpack_grad_kernel(const float* gp, uint32_t n_out, uint32_t n_in, uint32_t ldk,
+                                   float* g) {
+    const uint64_t n = (uint64_t)n_out * n_in;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = (uint32_t)(t / n_in), k = (uint32_t)(t % n_in);
+        g[t] = gp[(size_t)o * ldk + k];
+    }
+}
+
+__global__ void zero_f32_kernel(float* p, uint64_t n) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        p[t] = 0.0f;
+}
+
+__global__ void zero_f64_kernel(double* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0.0;
+}
+
+unsigned grid1d(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
+}  // namespace
+
+struct pp_mlp {
+    pp_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t n_layers = 0;
+    uint32_t dims[kMaxLayers + 1];      // true widths: dims[0] = n_in, dims[L] = n_out
+    uint32_t pdim[kMaxLayers + 1];      // padded to 16
+    uint64_t off_w[kMaxLayers], off_b[kMaxLayers], n_params = 0;
+    DevBuf w, grad, adam_m, adam_v;     // fp32 master / grads / Adam state (packed)
+    DevBuf gpad;                        // fp32 padded weight-gradient scratch
+    DevBuf wt16[kMaxLayers], w16[kMaxLayers];
+    // activations (grow with batch)
+    uint32_t cap_rows = 0;
+    DevBuf x16, act[kMaxLayers + 1], actT[kMaxLayers + 1], z[kMaxLayers + 1], dA, dZ, dZT, dY,
+        yout, loss;
+    uint64_t step = 0;
+};
+
+namespace {
+
+int mlp_reserve(pp_mlp* m, uint32_t rows) {
+    if (rows <= m->cap_rows) return PP_OK;
+    const uint32_t R = pad16(rows);
+    int rc;
+    const uint32_t L = m->n_layers;
+    uint32_t wmax = 16;
+    for (uint32_t l = 0; l <= L; ++l) wmax = std::max(wmax, m->pdim[l]);
+    if ((rc = ensure(m->x16, (size_t)R * m->pdim[0] * sizeof(bf16)))) return rc;
+    for (uint32_t l = 1; l < L; ++l) {
+        if ((rc = ensure(m->act[l], (size_t)R * m->pdim[l] * sizeof(bf16)))) return rc;
+        if ((rc = ensure(m->z[l], (size_t)R * m->pdim[l] * sizeof(float)))) return rc;
+        // rows beyond a batch's M are read (times zero gradients) by the weight
+        // gradient GEMM; start them finite
+        PP_HIP(hipMemsetAsync(m->act[l].p, 0, m->act[l].cap, m->stream));
+    }
+    for (uint32_t l = 0; l < L; ++l)
+        if ((rc = ensure(m->actT[l], (size_t)R * m->pdim[l] * sizeof(bf16)))) return rc;
+    if ((rc = ensure(m->dA, (size_t)R * wmax * sizeof(float)))) return rc;
+    if ((rc = ensure(m->dZ, (size_t)R * wmax * sizeof(bf16)))) return rc;
+    if ((rc = ensure(m->dZT, (size_t)R * wmax * sizeof(bf16)))) return rc;
+    if ((rc = ensure(m->dY, (size_t)R * m->pdim[L] * sizeof(bf16)))) return rc;
+    if ((rc = ensure(m->yout, (size_t)R * m->pdim[L] * sizeof(float)))) return rc;
+    if ((rc = ensure(m->loss, 2 * sizeof(double)))) return rc;
+    m->cap_rows = R;
+    return PP_OK;
+}
+
+int gemm(pp_mlp* m, const bf16* A, uint32_t lda, const bf16* B, uint32_t ldb, uint32_t M,
+         uint32_t N, uint32_t K, int epi, float* C, uint32_t ldc, const float* bias, bf16* Y,
+         uint32_t ldy, uint32_t splits = 1) {
+    GemmArgs g;
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.bias = bias;
+    g.Y = Y;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.ldy = ldy;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    const uint32_t per = ((K + splits - 1) / splits + 15) & ~15u;
+    g.k_split = per;
+    g.epi = epi;
+    dim3 grid((N + 127) / 128, (M + 127) / 128, (K + per - 1) / per);
+    gemm_nt_kernel<<<grid, 256, 0, m->stream>>>(g);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}
+
+int sync_weights(pp_mlp* m) {
+    for (uint32_t l = 0; l < m->n_layers; ++l) {
+        const uint32_t ni = m->dims[l], no = m->dims[l + 1];
+        const uint32_t ldk = m->pdim[l], ldn = m->pdim[l + 1];
+        sync_weights_kernel<<<grid1d((uint64_t)ldn * ldk), 256, 0, m->stream>>>(
+            (const float*)m->w.p + m->off_w[l], no, ni, (bf16*)m->wt16[l].p, ldk,
+            (bf16*)m->w16[l].p, ldn, ldk, ldn);
+    }
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}
+
+// Forward over M rows already packed in x16.  With `train`, keep z and act.
+int forward(pp_mlp* m, uint32_t M, bool train, float* out, uint32_t ld_out) {
+    const uint32_t L = m->n_layers;
+    const bf16* a = (const bf16*)m->x16.p;
+    int rc;
+    for (uint32_t l = 0; l < L; ++l) {
+        const uint32_t K = m->pdim[l], N = m->dims[l + 1];
+        const float* b = (const float*)m->w.p + m->off_b[l];
+        if (l + 1 < L) {
+            bf16* y = (bf16*)m->act[l + 1].p;
+            if (train)
+                rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_BIAS_SILU,
+                          (float*)m->z[l + 1].p, m->pdim[l + 1], b, y, m->pdim[l + 1]);
+            else
+                rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_SILU_ONLY, nullptr, 0,
+                          b, y, m->pdim[l + 1]);
+            if (rc) return rc;
+            a = y;
+        } else {
+            if ((rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_BIAS, out, ld_out, b,
+                           nullptr, 0)))
+                return rc;
+        }
+    }
+    return PP_OK;
+}
+
+int pack_input(pp_mlp* m, const float* d_x, uint32_t M) {
+    const uint32_t R = pad16(M);
+    pack_input_kernel<<<grid1d((uint64_t)R * m->pdim[0]), 256, 0, m->stream>>>(
+        d_x, M, R, m->dims[0], (bf16*)m->x16.p, m->pdim[0]);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}
+
+// Loss and gradients into m->grad (packed, fp32) for M rows in x16.
+int loss_grad(pp_mlp* m, uint32_t M, const float* d_logn, const float* d_k, const float* d_times,
+              uint32_t n_times, float time_weight) {
+    const uint32_t L = m->n_layers, R = pad16(M);
+    int rc;
+    const uint32_t ldo = m->pdim[L];
+    if ((rc = forward(m, M, true, (float*)m->yout.p, ldo))) return rc;
+    zero_f64_kernel<<<1, 64, 0, m->stream>>>((double*)m->loss.p, 2);
+    loss_kernel<<<(R + 255) / 256, 256, 0, m->stream>>>(
+        (const float*)m->yout.p, ldo, d_logn, d_k, d_times, n_times, M, R, time_weight,
+        (bf16*)m->dY.p, ldo, (double*)m->loss.p);
+    zero_f32_kernel<<<grid1d(m->n_params), 256, 0, m->stream>>>((float*)m->grad.p, m->n_params);
+    PP_HIP(hipGetLastError());
+    // transposed activations for the weight gradients: act_l^T [pdim_l][R]
+    {
+        const bf16* a0 = (const bf16*)m->x16.p;
+        for (uint32_t l = 0; l < L; ++l) {
+            const bf16* src = l == 0 ? a0 : (const bf16*)m->act[l].p;
+            dim3 g((m->pdim[l] + 31) / 32, (R + 31) / 32);
+            transpose_kernel<<<g, 256, 0, m->stream>>>(src, R, m->pdim[l], m->pdim[l],
+                                                       (bf16*)m->actT[l].p, R);
+        }
+    }
+    const bf16* dz = (const bf16*)m->dY.p;  // gradient w.r.t. the current layer's output
+    uint32_t ldz = ldo;
+    for (int l = (int)L - 1; l >= 0; --l) {
+        const uint32_t K = m->pdim[l], N = m->dims[l + 1], Np = m->pdim[l + 1];
+        // dZ^T [Np][R]
+        {
+            dim3 g((Np + 31) / 32, (R + 31) / 32);
+            transpose_kernel<<<g, 256, 0, m->stream>>>(dz, R, Np, ldz, (bf16*)m->dZT.p, R);
+        }
+        // dWt_l [N][K] += dZ^T · act_l  (inner dimension = batch)
+        float* gp = (float*)m->gpad.p;
+        zero_f32_kernel<<<grid1d((uint64_t)N * K), 256, 0, m->stream>>>(gp, (uint64_t)N * K);
+        const uint32_t splits = std::max(1u, std::min(64u, R / 256));
+        if ((rc = gemm(m, (const bf16*)m->dZT.p, R, (const bf16*)m->actT[l].p, R, N, K, R,
+                       EPI_ATOMIC, gp, K, nullptr, nullptr, 0, splits)))
+            return rc;
+        unpack_grad_kernel<<<grid1d((uint64_t)N * m->dims[l]), 256, 0, m->stream>>>(
+            gp, N, m->dims[l], K, (float*)m->grad.p + m->off_w[l]);
+        bias_grad_kernel<<<(N + 63) / 64, 256, 0, m->stream>>>(dz, M, N, ldz,
+                                                              (float*)m->grad.p + m->off_b[l]);
+        if (l == 0) break;
+        // dA_l = dZ · Wt_l  (B operand = W_l = Wt_l^T, [K][Np])
+        if ((rc = gemm(m, dz, ldz, (const bf16*)m->w16[l].p, Np, M, K, Np, EPI_STORE,
+                       (float*)m->dA.p, K, nullptr, nullptr, 0)))
+            return rc;
+        silu_backward_kernel<<<grid1d((uint64_t)R * K), 256, 0, m->stream>>>(
+            (const float*)m->dA.p, (const float*)m->z[l].p, M, R, K, K, (bf16*)m->dZ.p);
+        dz = (const bf16*)m->dZ.p;
+        ldz = K;
+    }
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_mlp_create(pp_ctx* ctx, uint32_t n_in, uint32_t width, uint32_t depth, uint32_t n_out,
+                  uint64_t seed, pp_mlp** out) {
+    if (!ctx || !out) return fail(PP_ERR_INVALID_ARG, "ctx/out is NULL");
+    *out = nullptr;
+    if (n_in < 1 || n_in > 4096 || width < 16 || width > 4096 || width % 16 || depth < 1 ||
+        depth > kMaxLayers - 1 || n_out < 1 || n_out > 16)
+        return fail(PP_ERR_INVALID_ARG,
+                    "unsupported MLP shape n_in=%u width=%u depth=%u n_out=%u (width multiple of "
+                    "16, depth <= %d, n_out <= 16)", n_in, width, depth, n_out, kMaxLayers - 1);
+    PP_HIP(hipSetDevice(ctx->device));
+    pp_mlp* m = new pp_mlp();
+    m->ctx = ctx;
+    m->n_layers = depth + 1;
+    m->dims[0] = n_in;
+    for (uint32_t l = 1; l <= depth; ++l) m->dims[l] = width;
+    m->dims[depth + 1] = n_out;
+    for (uint32_t l = 0; l <= m->n_layers; ++l) m->pdim[l] = pad16(m->dims[l]);
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < m->n_layers; ++l) {
+        m->off_w[l] = off;
+        off += (uint64_t)m->dims[l + 1] * m->dims[l];
+        m->off_b[l] = off;
+        off += m->dims[l + 1];
+    }
+    m->n_params = off;
+    int rc = PP_OK;
+    auto bail = [&](int code) {
+        pp_mlp_destroy(m);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
+        return bail(fail(PP_ERR_DEVICE, "stream/event creation failed"));
+    if ((rc = ensure(m->w, off * 4)) || (rc = ensure(m->grad, off * 4)) ||
+        (rc = ensure(m->adam_m, off * 4)) || (rc = ensure(m->adam_v, off * 4)))
+        return bail(rc);
+    uint64_t gmax = 0;
+    for (uint32_t l = 0; l < m->n_layers; ++l) {
+        gmax = std::max<uint64_t>(gmax, (uint64_t)m->dims[l + 1] * m->pdim[l]);
+        if ((rc = ensure(m->wt16[l], (size_t)m->pdim[l + 1] * m->pdim[l] * sizeof(bf16))) ||
+            (rc = ensure(m->w16[l], (size_t)m->pdim[l] * m->pdim[l + 1] * sizeof(bf16))))
+            return bail(rc);
+        if (hipMemset(m->wt16[l].p, 0, m->wt16[l].cap) != hipSuccess)
+            return bail(fail(PP_ERR_DEVICE, "hipMemset failed"));
+    }
+    if ((rc = ensure(m->gpad, gmax * 4))) return bail(rc);
+    // He-style uniform init, biases zero.
+    std::vector<float> h(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (uint32_t l = 0; l < m->n_layers; ++l) {
+        const float lim = std::sqrt(6.0f / (float)m->dims[l]) * (l + 1 == m->n_layers ? 0.1f : 1.0f);
+        std::uniform_real_distribution<float> U(-lim, lim);
+        for (uint64_t i = 0; i < (uint64_t)m->dims[l + 1] * m->dims[l]; ++i) h[m->off_w[l] + i] = U(rng);
+    }
+    if (hipMemcpy(m->w.p, h.data(), off * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(m->adam_m.p, 0, off * 4) != hipSuccess ||
+        hipMemset(m->adam_v.p, 0, off * 4) != hipSuccess)
+        return bail(fail(PP_ERR_DEVICE, "weight upload failed"));
+    if ((rc = sync_weights(m))) return bail(rc);
+    if (hipStreamSynchronize(m->stream) != hipSuccess)
+        return bail(fail(PP_ERR_DEVICE, "stream sync failed"));
+    *out = m;
+    return PP_OK;
+}
+
+int pp_mlp_destroy(pp_mlp* m) {
+    if (!m) return PP_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    DevBuf* bufs[] = {&m->w, &m->grad, &m->adam_m, &m->adam_v, &m->gpad, &m->x16, &m->dA,
+                      &m->dZ, &m->dZT, &m->dY, &m->yout, &m->loss};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (int l = 0; l <= kMaxLayers; ++l) {
+        if (l < kMaxLayers) {
+            if (m->wt16[l].p) (void)hipFree(m->wt16[l].p);
+            if (m->w16[l].p) (void)hipFree(m->w16[l].p);
+        }
+        if (m->act[l].p) (void)hipFree(m->act[l].p);
+        if (m->actT[l].p) (void)hipFree(m->actT[l].p);
+        if (m->z[l].p) (void)hipFree(m->z[l].p);
+    }
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return PP_OK;
+}
+
+int pp_mlp_n_params(pp_mlp* m, uint64_t* n) {
+    if (!m || !n) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    *n = m->n_params;
+    return PP_OK;
+}
+
+int pp_mlp_get_params(pp_mlp* m, float* host, uint64_t n) {
+    if (!m || !host || n != m->n_params)
+        return fail(PP_ERR_INVALID_ARG, "params buffer must hold exactly %llu floats",
+                    m ? (unsigned long long)m->n_params : 0ull);
+    PP_HIP(hipSetDevice(m->ctx->device));
+    PP_HIP(hipMemcpyAsync(host, m->w.p, n * 4, hipMemcpyDeviceToHost, m->stream));
+    PP_HIP(hipStreamSynchronize(m->stream));
+    return PP_OK;
+}
+
+int pp_mlp_set_params(pp_mlp* m, const float* host, uint64_t n) {
+    if (!m || !host || n != m->n_params)
+        return fail(PP_ERR_INVALID_ARG, "params buffer must hold exactly %llu floats",
+                    m ? (unsigned long long)m->n_params : 0ull);
+    PP_HIP(hipSetDevice(m->ctx->device));
+    PP_HIP(hipMemcpyAsync(m->w.p, host, n * 4, hipMemcpyHostToDevice, m->stream));
+    int rc = sync_weights(m);
+    if (rc) return rc;
+    PP_HIP(hipStreamSynchronize(m->stream));
+    return PP_OK;
+}
+
+int pp_mlp_forward_device(pp_mlp* m, const float* d_x, uint64_t n_rows, float* d_y) {
+    if (!m || (!d_x && n_rows) || (!d_y && n_rows)) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    if (n_rows == 0) return PP_OK;
+    if (n_rows > (1ull << 26)) return fail(PP_ERR_INVALID_ARG, "at most 2^26 rows per call");
+    PP_HIP(hipSetDevice(m->ctx->device));
+    const uint32_t M = (uint32_t)n_rows;
+    int rc;
+    if ((rc = mlp_reserve(m, M))) return rc;
+    if ((rc = pack_input(m, d_x, M))) return rc;
+    PP_HIP(hipEventRecord(m->ev0, m->stream));
+    const uint32_t L = m->n_layers;
+    // outputs straight into d_y [M][n_out]
+    if ((rc = forward(m, M, false, (float*)m->yout.p, m->pdim[L]))) return rc;
+    PP_HIP(hipEventRecord(m->ev1, m->stream));
+    PP_HIP(hipMemcpy2DAsync(d_y, m->dims[L] * 4, m->yout.p, m->pdim[L] * 4, m->dims[L] * 4, M,
+                            hipMemcpyDeviceToDevice, m->stream));
+    PP_HIP(hipStreamSynchronize(m->stream));
+    return PP_OK;
+}
+
+int pp_mlp_forward(pp_mlp* m, const float* x, uint64_t n_rows, float* y) {
+    if (!m || (!x && n_rows) || (!y && n_rows)) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    if (n_rows == 0) return PP_OK;
+    PP_HIP(hipSetDevice(m->ctx->device));
+    void *dx = nullptr, *dy = nullptr;
+    PP_HIP(hipMalloc(&dx, n_rows * m->dims[0] * 4));
+    if (hipMalloc(&dy, n_rows * m->dims[m->n_layers] * 4) != hipSuccess) {
+        (void)hipFree(dx);
+        return fail(PP_ERR_OOM, "hipMalloc failed");
+    }
+    int rc = PP_OK;
+    if (hipMemcpy(dx, x, n_rows * m->dims[0] * 4, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(PP_ERR_DEVICE, "copy failed");
+    if (!rc) rc = pp_mlp_forward_device(m, (const float*)dx, n_rows, (float*)dy);
+    if (!rc && hipMemcpy(y, dy, n_rows * m->dims[m->n_layers] * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(PP_ERR_DEVICE, "copy failed");
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return rc;
+}
+
+int pp_mlp_loss_grad_device(pp_mlp* m, const float* d_x, const float* d_logn, const float* d_k,
+                            const float* d_times, uint32_t n_times, uint32_t n_rows,
+                            float time_weight, float* grads_host, double* loss_out) {
+    if (!m || !d_x || !d_logn || !d_k || (n_times && !d_times) || n_rows == 0)
+        return fail(PP_ERR_INVALID_ARG, "NULL argument or empty batch");
+    PP_HIP(hipSetDevice(m->ctx->device));
+    int rc;
+    if ((rc = mlp_reserve(m, n_rows))) return rc;
+    if ((rc = pack_input(m, d_x, n_rows))) return rc;
+    if ((rc = loss_grad(m, n_rows, d_logn, d_k, d_times, n_times, time_weight))) return rc;
+    if (grads_host)
+        PP_HIP(hipMemcpyAsync(grads_host, m->grad.p, m->n_params * 4, hipMemcpyDeviceToHost,
+                              m->stream));
+    double l[2];
+    PP_HIP(hipMemcpyAsync(l, m->loss.p, sizeof l, hipMemcpyDeviceToHost, m->stream));
+    PP_HIP(hipStreamSynchronize(m->stream));
+    if (loss_out) {
+        loss_out[0] = l[0] / n_rows;
+        loss_out[1] = l[1] / n_rows;
+    }
+    return PP_OK;
+}
+
+int pp_mlp_train_step_device(pp_mlp* m, const float* d_x, const float* d_logn, const float* d_k,
+                             const float* d_times, uint32_t n_times, uint32_t n_rows, float lr,
+                             float time_weight, double* loss_out) {
+    int rc = pp_mlp_loss_grad_device(m, d_x, d_logn, d_k, d_times, n_times, n_rows, time_weight,
+                                     nullptr, loss_out);
+    if (rc) return rc;
+    const float b1 = 0.9f, b2 = 0.999f;
+    ++m->step;
+    const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
+    adam_kernel<<<grid1d(m->n_params), 256, 0, m->stream>>>(
+        (float*)m->w.p, (const float*)m->grad.p, (float*)m->adam_m.p, (float*)m->adam_v.p,
+        m->n_params, lr, b1, b2, 1e-8f, c1, c2);
+    PP_HIP(hipGetLastError());
+    if ((rc = sync_weights(m))) return rc;
+    PP_HIP(hipStreamSynchronize(m->stream));
+    return PP_OK;
+}
+
+int pp_mlp_last_forward_ms(pp_mlp* m, float* ms) {
+    if (!m || !ms) return fail(PP_ERR_INVALID_ARG, "NULL argument");
+    PP_HIP(hipEventElapsedTime(ms, m->ev0, m->ev1));
+    return PP_OK;
+}
+
+}  // extern "C"

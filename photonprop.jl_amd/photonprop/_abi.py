@@ -154,9 +154,10 @@ SIGNATURES = {
                                     ctypes.POINTER(pp_sphere), _u64, _u32, _u32, _vp, _u64,
                                     ctypes.POINTER(pp_summary)]),
     "pp_propagate_batch": (ctypes.c_int, [_vp, ctypes.POINTER(pp_request), _u32,
-                                          ctypes.POINTER(pp_spectrum), _vp, _u64, _vp, _vp]),
+                                          ctypes.POINTER(pp_spectrum), _vp, _u64, _u64, _vp, _vp]),
     "pp_propagate_batch_device": (ctypes.c_int, [_vp, ctypes.POINTER(pp_request), _u32,
-                                                 ctypes.POINTER(pp_spectrum), _vp, _u64, _vp, _vp]),
+                                                 ctypes.POINTER(pp_spectrum), _vp, _u64, _u64, _vp,
+                                                 _vp]),
     "pp_synchronize": (ctypes.c_int, [_vp]),
     "pp_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "pp_server_create": (ctypes.c_int, [_vp, ctypes.POINTER(pp_spectrum), ctypes.POINTER(pp_server_config),
@@ -167,6 +168,18 @@ SIGNATURES = {
                                         ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_double)]),
     "pp_server_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(pp_server_stats)]),
     "pp_server_destroy": (ctypes.c_int, [_vp]),
+    "pp_mlp_create": (ctypes.c_int, [_vp, _u32, _u32, _u32, _u32, _u64, ctypes.POINTER(_vp)]),
+    "pp_mlp_destroy": (ctypes.c_int, [_vp]),
+    "pp_mlp_n_params": (ctypes.c_int, [_vp, ctypes.POINTER(_u64)]),
+    "pp_mlp_get_params": (ctypes.c_int, [_vp, _vp, _u64]),
+    "pp_mlp_set_params": (ctypes.c_int, [_vp, _vp, _u64]),
+    "pp_mlp_forward_device": (ctypes.c_int, [_vp, _vp, _u64, _vp]),
+    "pp_mlp_forward": (ctypes.c_int, [_vp, _vp, _u64, _vp]),
+    "pp_mlp_last_forward_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "pp_mlp_loss_grad_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, ctypes.c_float,
+                                               _vp, _vp]),
+    "pp_mlp_train_step_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, ctypes.c_float,
+                                                ctypes.c_float, _vp]),
     "pp_selftest_rotate": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64]),
     "pp_selftest_hg": (ctypes.c_int, [_vp, ctypes.c_float, _u64, _vp, _u64]),
     "pp_selftest_philox": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64]),

@@ -164,21 +164,25 @@ class Context:
             _abi.check(rc, self.lib)
         return PropagationResult(hits[: summ.n_stored].copy(), summary_dict(summ))
 
-    def propagate_batch(self, requests, spectrum, hits_cap=None, allow_overflow=False):
+    def propagate_batch(self, requests, spectrum, hits_cap=None, allow_overflow=False,
+                        per_request_cap=0):
         """Propagate many requests in one persistent launch (pp_propagate_batch).
         `requests` is a REQUEST_DTYPE array.  Returns (hits, offsets, summaries):
-        request r owns hits[offsets[r]:offsets[r+1]]."""
+        request r owns hits[offsets[r]:offsets[r+1]].  per_request_cap > 0 keeps
+        at most that many records per request."""
         reqs = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
         n = reqs.shape[0]
         cap = int(min(int(reqs["n_photons"].sum()), DEFAULT_HITS_CAP)
                   if hits_cap is None else hits_cap)
-        hits = np.empty(cap, dtype=_abi.HIT_DTYPE)
+        out_cap = min(cap, per_request_cap * n) if per_request_cap else cap
+        hits = np.empty(out_cap, dtype=_abi.HIT_DTYPE)
         offsets = np.zeros(n + 1, dtype=np.uint64)
         sums = np.zeros(n, dtype=_abi.SUMMARY_DTYPE)
         spec, keep = spectrum_struct(spectrum)
         rc = self.lib.pp_propagate_batch(
             self.handle, reqs.ctypes.data_as(ctypes.POINTER(_abi.pp_request)), n,
-            ctypes.byref(spec), _abi.ptr(hits), cap, _abi.ptr(offsets), _abi.ptr(sums))
+            ctypes.byref(spec), _abi.ptr(hits) if out_cap else None, cap, int(per_request_cap),
+            _abi.ptr(offsets), _abi.ptr(sums))
         del keep
         if not (rc == _abi.PP_ERR_OVERFLOW and allow_overflow):
             _abi.check(rc, self.lib)

@@ -112,3 +112,19 @@ def test_batched_4096_pairs_workload(ctx):
     assert steps > 9.0 * int(reqs["n_photons"].sum())
     print("4096 pairs: photons %.3g steps %.3g hits %d, kernel %.1f ms"
           % (reqs["n_photons"].sum(), steps, hits.shape[0], ctx.last_kernel_ms()))
+
+
+def test_per_request_cap_keeps_a_subset(ctx):
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = mixed_batch(20, seed=6)
+    full_h, full_o, full_s = ctx.propagate_batch(reqs, spec)
+    h, o, s = ctx.propagate_batch(reqs, spec, per_request_cap=3)
+    for k in SUM_KEYS:
+        np.testing.assert_array_equal(s[k], full_s[k])
+    np.testing.assert_array_equal(s["n_stored"], np.minimum(full_s["n_hits"], 3))
+    assert np.all(s["status"] == 0)
+    for r in range(len(reqs)):
+        seg = h[int(o[r]):int(o[r + 1])]
+        full = full_h[int(full_o[r]):int(full_o[r + 1])]
+        assert seg.shape[0] == min(3, full.shape[0])
+        assert set(seg["photon_id"].tolist()) <= set(full["photon_id"].tolist())
