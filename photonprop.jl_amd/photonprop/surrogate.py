@@ -1,0 +1,298 @@
+"""MLP surrogate of the propagation (the role of PhotonProp.jl's Modelling
+module, src/modelling.jl).
+
+The reference fits a weighted Gamma distribution per (distance, angle) cell of
+its MC output (fit_photon_dist, :52-79), transforms the inputs to (log distance,
+cos angle) (source_to_input, :269-289) and trains a small Flux MLP on the fitted
+parameters with an MSE loss (train_mlp, :202-267).  Here the MLP is trained
+directly on MC rows with the likelihoods themselves:
+
+  input (8)   log10 distance, cos(emitter axis, target direction),
+              cos(target axis +z, direction to the emitter), target radius,
+              four medium scalars (absorption length at 450 nm, scattering length
+              at 450 nm, HG g, refractive-index offset), each affinely scaled
+  output (10) y0 = log expected yield per emitted photon,
+              3-component gamma mixture of the residual arrival time
+              (logits, log shapes, log rates per 10 ns)
+  loss        Poisson NLL of the summed absorption weight of each MC row +
+              NLL of up to T sampled residual times per row
+
+Training, inference and the loss run in libphotonprop.so (pp_mlp_*:
+hand-written BF16 MFMA kernels, FP32 accumulation).  Queries outside the
+training domain, or where the predicted yield is below the floor the training
+MC could resolve, fall back to the plain MC kernel.
+"""
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .medium import SeawaterConfigMedium
+from .propagation import C_VACUUM, make_requests
+from .spectrum import InverseSquareSpectrum
+
+N_IN, WIDTH, DEPTH, N_OUT = 8, 256, 4, 10
+
+# Training domain and affine scaling of each feature: (lo, hi).
+DOMAIN = {
+    "log10_distance": (0.0, math.log10(200.0)),
+    "cos_axis": (-1.0, 1.0),
+    "cos_view": (-1.0, 1.0),
+    "radius": (0.2, 0.5),
+    "abs_450": (20.0, 40.0),
+    "sca_450": (30.0, 70.0),
+    "g": (0.85, 0.95),
+    "n_offset": (-0.01, 0.01),
+}
+FEATURES = list(DOMAIN)
+
+
+def scale_features(raw):
+    """raw (n, 8) in physical units -> inputs in [-1, 1]."""
+    raw = np.asarray(raw, dtype=np.float64)
+    lo = np.array([DOMAIN[k][0] for k in FEATURES])
+    hi = np.array([DOMAIN[k][1] for k in FEATURES])
+    return (2.0 * (raw - lo) / (hi - lo) - 1.0).astype(np.float32)
+
+
+def in_domain(raw, slack=1e-6):
+    raw = np.asarray(raw, dtype=np.float64)
+    lo = np.array([DOMAIN[k][0] for k in FEATURES]) - slack
+    hi = np.array([DOMAIN[k][1] for k in FEATURES]) + slack
+    return np.all((raw >= lo) & (raw <= hi), axis=1)
+
+
+def medium_for(abs_450, sca_450, g, n_offset):
+    """The seawater-like medium with its four scalars varied."""
+    return SeawaterConfigMedium(n_300=1.36 + n_offset, n_600=1.34 + n_offset, abs_peak=abs_450,
+                                abs_edge=abs_450 / 3.0, sca_450=sca_450, hg_g=g)
+
+
+def t_geo(distance, radius, medium):
+    """Earliest possible arrival: the straight path at the largest phase speed."""
+    n_min = float(min(medium.refractive_index(300.0), medium.refractive_index(600.0)))
+    return (np.asarray(distance) - np.asarray(radius)) * n_min / C_VACUUM
+
+
+def random_rows(rng, n, medium_scalars):
+    """n random geometries for one medium: emitter at the origin with a random
+    axis, target along a random direction at a log-uniform distance, random
+    target axis... the raw feature matrix and matching requests."""
+    ld = rng.uniform(*DOMAIN["log10_distance"], n)
+    d = 10.0 ** ld
+    radius = rng.uniform(*DOMAIN["radius"], n)
+    # target direction u (unit), emitter axis a with cos(a, u) uniform
+    z = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * math.pi, n)
+    u = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], 1)
+    ca = rng.uniform(-1, 1, n)
+    # any unit vector perpendicular to u
+    t = np.cross(u, np.where(np.abs(u[:, :1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]]))
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    axis = ca[:, None] * u + np.sqrt(1 - ca * ca)[:, None] * t
+    cos_view = -u[:, 2]  # target axis +z against the direction target -> emitter (-u)
+    raw = np.zeros((n, 8))
+    raw[:, 0], raw[:, 1], raw[:, 2], raw[:, 3] = ld, ca, cos_view, radius
+    raw[:, 4:] = medium_scalars
+    reqs = make_requests(n)
+    reqs["center"] = u * d[:, None]
+    reqs["axis"] = axis
+    reqs["radius"] = radius
+    reqs["beta"] = 1.0
+    reqs["profile"] = _abi.PP_EMIT_CHERENKOV_CONE
+    return raw, reqs
+
+
+@dataclass
+class TrainingSet:
+    raw: np.ndarray        # (n, 8) physical features
+    x: np.ndarray          # (n, 8) scaled
+    log_n: np.ndarray      # (n,) log photons per row
+    k: np.ndarray          # (n,) summed absorption weight
+    times: np.ndarray      # (n, T) residual times in ns, -1 where absent
+    steps: int = 0
+    photons: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def generate_training_set(ctx, n_rows, photons_per_row=200_000, rows_per_medium=4096, n_times=16,
+                          seed=0, spectrum=None):
+    """MC rows from the batched kernel: each group of rows_per_medium rows shares
+    a randomly drawn medium; each row is one request of photons_per_row photons
+    with up to n_times hit times kept (a subset independent of the photons'
+    histories)."""
+    spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
+    rng = np.random.default_rng(seed)
+    raws, lns, ks, ts = [], [], [], []
+    steps = 0
+    done = 0
+    batch = 0
+    while done < n_rows:
+        n = min(rows_per_medium, n_rows - done)
+        ms = np.array([rng.uniform(*DOMAIN[k]) for k in ("abs_450", "sca_450", "g", "n_offset")])
+        medium = medium_for(*ms)
+        ctx.set_medium(medium)
+        raw, reqs = random_rows(rng, n, ms)
+        reqs["n_photons"] = photons_per_row
+        reqs["seed"] = seed * 1_000_003 + batch
+        reqs["stream"] = np.arange(n, dtype=np.uint32)
+        hits, off, sums = ctx.propagate_batch(reqs, spectrum, per_request_cap=n_times,
+                                              hits_cap=int(min(photons_per_row * n, 1 << 27)),
+                                              allow_overflow=True)
+        tg = t_geo(np.linalg.norm(reqs["center"], axis=1), reqs["radius"], medium)
+        times = -np.ones((n, n_times), dtype=np.float32)
+        cnt = np.diff(off.astype(np.int64))
+        row = np.repeat(np.arange(n), cnt)
+        slot = np.arange(hits.shape[0]) - np.repeat(off[:-1].astype(np.int64), cnt)
+        times[row, slot] = np.maximum(hits["time"].astype(np.float64) - tg[row], 0.0)
+        raws.append(raw)
+        lns.append(np.full(n, math.log(photons_per_row), dtype=np.float32))
+        ks.append(sums["sum_w"].astype(np.float32))
+        ts.append(times)
+        steps += int(sums["n_steps"].sum())
+        done += n
+        batch += 1
+    raw = np.concatenate(raws)
+    return TrainingSet(raw=raw, x=scale_features(raw), log_n=np.concatenate(lns),
+                       k=np.concatenate(ks), times=np.concatenate(ts), steps=steps,
+                       photons=int(photons_per_row) * n_rows)
+
+
+class MLP:
+    """The 8 -> 256^4 -> 10 surrogate network on the GPU (pp_mlp_*)."""
+
+    def __init__(self, ctx, seed=0, n_in=N_IN, width=WIDTH, depth=DEPTH, n_out=N_OUT):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n_in, self.n_out = n_in, n_out
+        self._h = ctypes.c_void_p()
+        _abi.check(self.lib.pp_mlp_create(ctx.handle, n_in, width, depth, n_out, int(seed),
+                                          ctypes.byref(self._h)), self.lib)
+        n = ctypes.c_uint64()
+        _abi.check(self.lib.pp_mlp_n_params(self._h, ctypes.byref(n)), self.lib)
+        self.n_params = n.value
+
+    def close(self):
+        if self._h:
+            self.lib.pp_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        _abi.check(self.lib.pp_mlp_get_params(self._h, _abi.ptr(p), self.n_params), self.lib)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _abi.check(self.lib.pp_mlp_set_params(self._h, _abi.ptr(p), self.n_params), self.lib)
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((x.shape[0], self.n_out), dtype=np.float32)
+        _abi.check(self.lib.pp_mlp_forward(self._h, _abi.ptr(x), x.shape[0], _abi.ptr(y)), self.lib)
+        return y
+
+    def forward_device(self, d_x, d_y, n):
+        """Device pointers (ints, e.g. torch data_ptr()); returns kernel ms."""
+        _abi.check(self.lib.pp_mlp_forward_device(self._h, ctypes.c_void_p(d_x), int(n),
+                                                  ctypes.c_void_p(d_y)), self.lib)
+        ms = ctypes.c_float()
+        _abi.check(self.lib.pp_mlp_last_forward_ms(self._h, ctypes.byref(ms)), self.lib)
+        return ms.value
+
+    def loss_grad(self, d_x, d_logn, d_k, d_times, n_times, n, time_weight=1.0, want_grad=True):
+        g = np.empty(self.n_params, dtype=np.float32) if want_grad else None
+        loss = np.zeros(2)
+        _abi.check(self.lib.pp_mlp_loss_grad_device(
+            self._h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_logn), ctypes.c_void_p(d_k),
+            ctypes.c_void_p(d_times), int(n_times), int(n), float(time_weight),
+            _abi.ptr(g) if want_grad else None, _abi.ptr(loss)), self.lib)
+        return loss, g
+
+    def train_step(self, d_x, d_logn, d_k, d_times, n_times, n, lr, time_weight=1.0):
+        loss = np.zeros(2)
+        _abi.check(self.lib.pp_mlp_train_step_device(
+            self._h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_logn), ctypes.c_void_p(d_k),
+            ctypes.c_void_p(d_times), int(n_times), int(n), float(lr), float(time_weight),
+            _abi.ptr(loss)), self.lib)
+        return loss
+
+
+def train(mlp, ts, epochs=8, batch=4096, lr=2e-3, time_weight=0.5, seed=0, log=None):
+    """Adam over shuffled minibatches held in device memory (torch tensors as
+    plumbing); cosine learning-rate decay.  Returns per-epoch mean losses."""
+    import torch
+
+    dev = "cuda:%d" % 0
+    X = torch.from_numpy(ts.x).to(dev)
+    LN = torch.from_numpy(ts.log_n).to(dev)
+    K = torch.from_numpy(ts.k).to(dev)
+    T = torch.from_numpy(ts.times).to(dev)
+    n = X.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    steps_total = epochs * (n // batch)
+    step = 0
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(dev)
+        acc = np.zeros(2)
+        nb = 0
+        for b in range(n // batch):
+            idx = perm[b * batch:(b + 1) * batch]
+            xb, lb, kb, tb = (X[idx].contiguous(), LN[idx].contiguous(), K[idx].contiguous(),
+                              T[idx].contiguous())
+            cur = lr * 0.5 * (1 + math.cos(math.pi * step / max(1, steps_total)))
+            acc += mlp.train_step(xb.data_ptr(), lb.data_ptr(), kb.data_ptr(), tb.data_ptr(),
+                                  T.shape[1], batch, cur, time_weight)
+            nb += 1
+            step += 1
+        hist.append((acc / max(nb, 1)).tolist())
+        if log:
+            log("epoch %d: poisson %.5f time %.5f" % (ep, hist[-1][0], hist[-1][1]))
+    return hist
+
+
+def gamma_mixture(y):
+    """(weights, shapes, rates per ns) from network outputs y[:, 1:10]."""
+    y = np.asarray(y, dtype=np.float64)
+    lg = y[:, 1:4] - y[:, 1:4].max(axis=1, keepdims=True)
+    w = np.exp(lg)
+    w /= w.sum(axis=1, keepdims=True)
+    a = np.exp(np.clip(y[:, 4:7], -4, 6))
+    b = np.exp(np.clip(y[:, 7:10], -8, 6)) / 10.0
+    return w, a, b
+
+
+class HybridYield:
+    """Yield per emitted photon from the surrogate, with the plain MC as the
+    fallback for rows out of the training domain or below the resolvable
+    yield floor."""
+
+    def __init__(self, mlp, ctx, yield_floor=1e-7, mc_photons=10_000_000, spectrum=None):
+        self.mlp = mlp
+        self.ctx = ctx
+        self.floor = yield_floor
+        self.mc_photons = mc_photons
+        self.spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
+
+    def query(self, raw, reqs, medium):
+        """raw (n, 8) physical features and the matching requests (same medium).
+        Returns (yield, used_mc mask)."""
+        y = self.mlp.forward(scale_features(raw))
+        yld = np.exp(y[:, 0].astype(np.float64))
+        use_mc = ~in_domain(raw) | (yld < self.floor)
+        if use_mc.any():
+            self.ctx.set_medium(medium)
+            r = reqs[use_mc].copy()
+            r["n_photons"] = self.mc_photons
+            _, _, sums = self.ctx.propagate_batch(r, self.spectrum, hits_cap=0, allow_overflow=True)
+            yld[use_mc] = sums["sum_w"] / self.mc_photons
+        return yld, use_mc

@@ -1,0 +1,177 @@
+"""MLP surrogate kernels (BF16 MFMA) against a plain PyTorch FP32 reference of
+the same network and loss, plus a short end-to-end train on MC rows."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import photonprop as pp
+from photonprop import surrogate as sg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def unpack(params, dims):
+    out, o = [], 0
+    for l in range(len(dims) - 1):
+        ni, no = dims[l], dims[l + 1]
+        W = torch.from_numpy(params[o:o + no * ni].reshape(no, ni).copy()); o += no * ni
+        b = torch.from_numpy(params[o:o + no].copy()); o += no
+        out.append((W, b))
+    return out
+
+
+def ref_forward(layers, x, requires_grad=False):
+    """FP32 reference with the kernels' BF16 rounding of weights and of each
+    layer's input activations."""
+    a = bf(torch.from_numpy(x))
+    params = []
+    for l, (W, b) in enumerate(layers):
+        Wb = bf(W).clone().requires_grad_(requires_grad)
+        bb = b.clone().requires_grad_(requires_grad)
+        params += [Wb, bb]
+        z = a @ Wb.T + bb
+        if l + 1 < len(layers):
+            s = z * torch.sigmoid(z)
+            a = s + (bf(s) - s).detach()  # BF16 activations, straight-through gradient
+        else:
+            return z, params
+
+
+def ref_loss(y, logn, k, times, tw):
+    y = y.double()
+    M = y.shape[0]
+    lmu = torch.clamp(y[:, 0] + logn, max=60.0)
+    lp = torch.exp(lmu) - k * lmu
+    logpi = torch.log_softmax(y[:, 1:4], dim=1)
+    la = torch.clamp(y[:, 4:7], -4, 6)
+    lb = torch.clamp(y[:, 7:10], -8, 6)
+    a, b = torch.exp(la), torch.exp(lb)
+    mask = times >= 0
+    ts = torch.where(mask, times, torch.zeros_like(times)) / 10.0 + 1e-3
+    lt_ = torch.log(ts)[:, :, None]
+    lpc = (logpi[:, None, :] + a[:, None, :] * lb[:, None, :] + (a[:, None, :] - 1) * lt_
+           - b[:, None, :] * ts[:, :, None] - torch.lgamma(a)[:, None, :])
+    ll = torch.logsumexp(lpc, dim=2)
+    nt = mask.sum(1)
+    lt = -(ll * mask).sum(1) / torch.clamp(nt, min=1)
+    return (lp.sum() + tw * lt.sum()) / M, lp.mean(), lt.mean()
+
+
+def batch(rng, M, T=8):
+    x = rng.uniform(-1, 1, (M, 8)).astype(np.float32)
+    logn = np.full(M, math.log(30.0), dtype=np.float32)
+    k = rng.poisson(30.0, M).astype(np.float32)
+    times = rng.gamma(2.0, 15.0, (M, T)).astype(np.float32)
+    times[rng.uniform(size=(M, T)) < 0.3] = -1.0
+    return x, logn, k, times
+
+
+def to_dev(*arrs):
+    return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrs]
+
+
+def test_forward_matches_torch_fp32(ctx):
+    mlp = sg.MLP(ctx, seed=1)
+    dims = [8, 256, 256, 256, 256, 10]
+    layers = unpack(mlp.get_params(), dims)
+    rng = np.random.default_rng(0)
+    for M in (1, 37, 1000, 4099):
+        x = rng.uniform(-1, 1, (M, 8)).astype(np.float32)
+        y = mlp.forward(x)
+        ref, _ = ref_forward(layers, x)
+        ref = ref.detach().numpy()
+        err = np.abs(y - ref).max() / max(np.abs(ref).max(), 1e-6)
+        say("forward M=%d max rel err %.2e" % (M, err))
+        assert err < 2e-2
+    mlp.close()
+
+
+def test_loss_and_gradients_match_torch_autograd(ctx):
+    mlp = sg.MLP(ctx, seed=2)
+    dims = [8, 256, 256, 256, 256, 10]
+    # make the output layer non-trivial so every mixture term gets gradient
+    p = mlp.get_params()
+    rng = np.random.default_rng(3)
+    p[-(10 * 256 + 10):] = rng.normal(0, 0.05, 10 * 256 + 10).astype(np.float32)
+    mlp.set_params(p)
+    layers = unpack(mlp.get_params(), dims)
+    M, T = 512, 8
+    x, logn, k, times = batch(rng, M, T)
+    dx, dln, dk, dt = to_dev(x, logn, k, times)
+    tw = 0.7
+    loss, g = mlp.loss_grad(dx.data_ptr(), dln.data_ptr(), dk.data_ptr(), dt.data_ptr(), T, M, tw)
+    y, params = ref_forward(layers, x, requires_grad=True)
+    L, lp, lt = ref_loss(y, torch.from_numpy(logn).double(), torch.from_numpy(k).double(),
+                         torch.from_numpy(times).double(), tw)
+    L.backward()
+    say("loss lib", loss, "ref", float(lp), float(lt))
+    assert loss[0] == pytest.approx(float(lp), rel=2e-2, abs=1e-3)
+    assert loss[1] == pytest.approx(float(lt), rel=2e-2, abs=1e-3)
+    o = 0
+    for l, (W, b) in enumerate(layers):
+        for ref_g in (params[2 * l].grad, params[2 * l + 1].grad):
+            n = ref_g.numel()
+            gl = torch.from_numpy(g[o:o + n]).double()
+            gr = ref_g.reshape(-1).double()
+            o += n
+            cos = float(gl @ gr / (gl.norm() * gr.norm() + 1e-30))
+            rel = float((gl.norm() - gr.norm()).abs() / (gr.norm() + 1e-30))
+            say("layer %d grad n=%d cos %.5f |rel norm| %.3e" % (l, n, cos, rel))
+            assert cos > 0.99 and rel < 0.05
+    assert o == mlp.n_params
+    mlp.close()
+
+
+def test_training_fits_a_known_yield_function(ctx):
+    """Poisson rows with log-yield f(x) = -9 + 2 x0 - x1²: the Poisson NLL must
+    approach its floor and the fitted log yield f within 0.1."""
+    rng = np.random.default_rng(5)
+    M = 65536
+    x = rng.uniform(-1, 1, (M, 8)).astype(np.float32)
+    f = -9.0 + 2.0 * x[:, 0] - x[:, 1] ** 2
+    logn = np.full(M, math.log(1e6), dtype=np.float32)
+    k = rng.poisson(np.exp(f + logn)).astype(np.float32)
+    times = -np.ones((M, 4), dtype=np.float32)
+    ts = sg.TrainingSet(raw=x, x=x, log_n=logn, k=k, times=times)
+    mlp = sg.MLP(ctx, seed=4)
+    hist = sg.train(mlp, ts, epochs=6, batch=2048, lr=3e-3, time_weight=0.0, log=say)
+    assert hist[-1][0] < hist[0][0]
+    xt = rng.uniform(-0.9, 0.9, (4096, 8)).astype(np.float32)
+    ft = -9.0 + 2.0 * xt[:, 0] - xt[:, 1] ** 2
+    y = mlp.forward(xt)
+    err = np.median(np.abs(y[:, 0] - ft))
+    say("median |log-yield error| %.4f" % err)
+    assert err < 0.1
+    mlp.close()
+
+
+def test_mc_rows_train_and_hybrid_fallback():
+    """A small MC training set from the batched kernel, a short training run,
+    and the MC fallback for an out-of-domain query."""
+    with pp.Context(pp.SeawaterConfigMedium()) as c:
+        ts = sg.generate_training_set(c, 16384, photons_per_row=20_000, rows_per_medium=4096,
+                                      n_times=8, seed=1)
+        assert ts.x.shape == (16384, 8) and np.all(np.abs(ts.x) <= 1.0 + 1e-6)
+        assert (ts.times >= 0).any() and ts.k.sum() > 0
+        mlp = sg.MLP(c, seed=0)
+        hist = sg.train(mlp, ts, epochs=3, batch=2048, lr=2e-3, log=say)
+        assert hist[-1][0] < hist[0][0]
+        hy = sg.HybridYield(mlp, c, mc_photons=100_000)
+        rng = np.random.default_rng(9)
+        ms = np.array([30.0, 50.0, 0.9, 0.0])
+        raw, reqs = sg.random_rows(rng, 8, ms)
+        raw[0, 0] = 3.0  # 1 km: outside the training domain
+        reqs["seed"] = 1
+        y, used = hy.query(raw, reqs, sg.medium_for(*ms))
+        assert used[0] and np.all(np.isfinite(y)) and np.all(y >= 0)
+        mlp.close()
