@@ -153,6 +153,92 @@ def bench_serving(pp, ctx, spectrum, n_req):
     return out
 
 
+def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, n_eval=512, eval_photons=20_000_000):
+    """Train the BF16-MFMA surrogate on MC rows from the batched kernel, then judge
+    it against high-statistics MC on held-out pairs: median relative yield error
+    (pairs with >= 400 weighted hits, i.e. <= 5% MC noise) and the median
+    Kolmogorov distance between the predicted gamma-mixture CDF and the MC
+    residual-time sample.  Speed: surrogate queries/s (one forward of 1e6 rows,
+    HIP events) against MC queries/s at the 4096-pair batch's mean budget."""
+    import torch
+    from scipy.special import gammainc
+
+    from photonprop import surrogate as sg
+    from photonprop import workloads
+
+    out = {}
+    t0 = time.perf_counter()
+    ts = sg.generate_training_set(ctx, n_rows, photons_per_row=photons_per_row, n_times=16, seed=3)
+    out["training_rows"] = n_rows
+    out["photons_per_row"] = photons_per_row
+    out["datagen_s"] = time.perf_counter() - t0
+    print("surrogate: %d rows generated in %.1f s" % (n_rows, out["datagen_s"]), file=sys.stderr,
+          flush=True)
+    mlp = sg.MLP(ctx, seed=0)
+    t0 = time.perf_counter()
+    hist = sg.train(mlp, ts, epochs=epochs, batch=4096, lr=2e-3, time_weight=0.5,
+                    log=lambda m: print("surrogate " + m, file=sys.stderr, flush=True))
+    out["train_s"] = time.perf_counter() - t0
+    out["epochs"] = epochs
+    out["final_loss"] = hist[-1]
+    del ts
+    # held-out evaluation in 4 media
+    rng = np.random.default_rng(1234)
+    yp, ym, ks_d, fallback = [], [], [], 0
+    per = n_eval // 4
+    for g in range(4):
+        ms = np.array([rng.uniform(*sg.DOMAIN[k]) for k in ("abs_450", "sca_450", "g", "n_offset")])
+        medium = sg.medium_for(*ms)
+        raw, reqs = sg.random_rows(rng, per, ms)
+        ctx.set_medium(medium)
+        reqs["n_photons"] = eval_photons
+        reqs["seed"] = 777 + g
+        reqs["stream"] = np.arange(per, dtype=np.uint32)
+        hits, off, sums = ctx.propagate_batch(reqs, pp.InverseSquareSpectrum(300.0, 600.0),
+                                              per_request_cap=2000, hits_cap=1 << 26,
+                                              allow_overflow=True)
+        y = mlp.forward(sg.scale_features(raw))
+        mc = sums["sum_w"] / eval_photons
+        good = sums["sum_w"] >= 400.0
+        yp.append(np.exp(y[good, 0].astype(np.float64)))
+        ym.append(mc[good])
+        w, a, b = sg.gamma_mixture(y)
+        tg = sg.t_geo(np.linalg.norm(reqs["center"], axis=1), reqs["radius"], medium)
+        for r in np.nonzero(good)[0]:
+            t = np.sort(hits["time"][int(off[r]):int(off[r + 1])].astype(np.float64) - tg[r])
+            t = np.maximum(t, 0.0) + 1e-2
+            if t.size < 50:
+                continue
+            cdf = (w[r][None, :] * gammainc(a[r][None, :], b[r][None, :] * t[:, None])).sum(1)
+            emp = np.arange(1, t.size + 1) / t.size
+            ks_d.append(float(np.max(np.abs(cdf - emp))))
+        fallback += int((np.exp(y[:, 0]) < 1e-7).sum() + (~sg.in_domain(raw)).sum())
+    yp, ym = np.concatenate(yp), np.concatenate(ym)
+    rel = np.abs(yp - ym) / ym
+    out["eval_pairs_resolved"] = int(rel.size)
+    out["median_yield_rel_error"] = float(np.median(rel))
+    out["p90_yield_rel_error"] = float(np.percentile(rel, 90))
+    out["median_time_ks_distance"] = float(np.median(ks_d)) if ks_d else None
+    out["eval_fallback_fraction"] = fallback / float(4 * per)
+    ctx.set_medium(pp.SeawaterConfigMedium())
+    # speed: surrogate forward on 1e6 queries vs MC per query
+    n_q = 1_000_000
+    xq = torch.rand((n_q, 8), device="cuda:0") * 2 - 1
+    yq = torch.empty((n_q, 10), device="cuda:0")
+    mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n_q)
+    ms = min(mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n_q) for _ in range(3))
+    out["surrogate_queries_per_s"] = n_q / (1e-3 * ms)
+    out["surrogate_tflops"] = n_q * 2.0 * (8 * 256 + 3 * 256 * 256 + 256 * 10) / (1e-3 * ms) / 1e12
+    reqs = workloads.batched_pairs(4096, seed=0)
+    ctx.propagate_batch(reqs, pp.InverseSquareSpectrum(300.0, 600.0), hits_cap=0, allow_overflow=True)
+    kms = ctx.last_kernel_ms()
+    out["mc_queries_per_s"] = 4096 / (1e-3 * kms)
+    out["mc_mean_photons_per_query"] = float(reqs["n_photons"].mean())
+    out["speedup_vs_mc"] = out["surrogate_queries_per_s"] / out["mc_queries_per_s"]
+    mlp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +251,10 @@ def main():
                     help="also measure the 4096-pair batch and the serving loop (0 skips)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests per arrival rate in the serving measurement")
+    ap.add_argument("--surrogate-rows", type=float, default=2e6,
+                    help="MC training rows for the surrogate measurement (0 skips it)")
+    ap.add_argument("--surrogate-photons", type=float, default=1e5, help="photons per training row")
+    ap.add_argument("--surrogate-epochs", type=int, default=10)
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter "
                          "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -280,6 +370,10 @@ def main():
         print("batched done", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(pp, ctx, spectrum, args.serve_requests)
         print("serving done", file=sys.stderr, flush=True)
+        if args.surrogate_rows > 0:
+            out["surrogate"] = bench_surrogate(pp, ctx, int(args.surrogate_rows),
+                                               int(args.surrogate_photons), args.surrogate_epochs)
+            print("surrogate done", file=sys.stderr, flush=True)
     if args.cpu_sample > 0:
         cb = cpu_baseline(pp, int(args.cpu_sample))
         out["cpu_baseline"] = cb

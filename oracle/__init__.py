@@ -52,6 +52,7 @@ def lib():
         L.or_philox4x32_10.argtypes = [vp, vp, vp]
         L.or_uniform.restype = ctypes.c_float
         L.or_uniform.argtypes = [ctypes.c_uint32]
+        L.or_six_uniforms.argtypes = [vp, vp]
         L.or_rodrigues_rotation.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -75,6 +76,13 @@ def philox4x32_10(counter, key):
 
 def uniform(word):
     return lib().or_uniform(int(word))
+
+
+def six_uniforms(words):
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    out = np.zeros(6, dtype=np.float32)
+    lib().or_six_uniforms(_p(w), _p(out))
+    return out
 
 
 def hg_costheta(g, eta, precision=32):

@@ -85,8 +85,20 @@ void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32
     out[3] = c3;
 }
 
-/* Uniform on (0, 1]: the top 24 bits plus one, times 2^-24 (exact in FP32). */
+/* Uniform on (0, 1]: the top 24 bits plus one, times 2^-24 (exact in FP32).
+ * Used by the device self-tests of single helpers. */
 float or_uniform(uint32_t w) { return fmaf((float)(w >> 8), 0x1p-24f, 0x1p-24f); }
+
+/* Six 21-bit uniforms on (0, 1] from one Philox block (the propagation's
+ * draws): v0..v3 are the top 21 bits of words 0..3; v4 joins the low 11 bits of
+ * word 0 with the low 10 bits of word 1, v5 likewise words 2 and 3; each
+ * u = (v + 1)·2^-21, exact in FP32. */
+void or_six_uniforms(const uint32_t w[4], float u[6]) {
+    uint32_t v[6] = {w[0] >> 11, w[1] >> 11, w[2] >> 11, w[3] >> 11,
+                     (w[0] & 0x7FFu) | ((w[1] & 0x3FFu) << 11),
+                     (w[2] & 0x7FFu) | ((w[3] & 0x3FFu) << 11)};
+    for (int i = 0; i < 6; ++i) u[i] = fmaf((float)v[i], 0x1p-21f, 0x1p-21f);
+}
 
 /* Rodrigues rotation taking a to b, applied to operand, in FP64
  * (src/photon_prop_cuda.jl:161-169). */

@@ -44,9 +44,20 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
     return U4{c0, c1, c2, c3};
 }
 
-// Uniform on (0, 1]: (top 24 bits + 1) * 2^-24, exact in FP32.
+// Uniform on (0, 1]: (top 24 bits + 1) * 2^-24, exact in FP32 (self-tests).
 __device__ __forceinline__ float u01(uint32_t w) {
     return __builtin_fmaf((float)(w >> 8), 0x1p-24f, 0x1p-24f);
+}
+
+// The propagation takes six 21-bit uniforms from each Philox block: the top 21
+// bits of words x, y, z, w, then (x & 0x7FF) | (y & 0x3FF) << 11 and
+// (z & 0x7FF) | (w & 0x3FF) << 11.  u21(v) = (v + 1) * 2^-21, exact in FP32.
+__device__ __forceinline__ float u21(uint32_t v) {
+    return __builtin_fmaf((float)v, 0x1p-21f, 0x1p-21f);
+}
+__device__ __forceinline__ uint32_t v21_hi(uint32_t w) { return w >> 11; }
+__device__ __forceinline__ uint32_t v21_lo(uint32_t a, uint32_t b) {
+    return (a & 0x7FFu) | ((b & 0x3FFu) << 11);
 }
 
 // Henyey-Greenstein cosθ by inverse CDF (src/photon_prop_cuda.jl:31-38):

@@ -8,10 +8,10 @@
 //   * One lane per photon.  A 256-lane block walks its tile in rounds of 256
 //     photons; every lane runs its photon to the hit or to max_steps.
 //   * Random numbers: Philox4x32-10 keyed by the request seed, counter
-//     (block, photon_lo, photon_hi, stream).  Block 0 holds the emission draws
-//     {λ, φ, θ, -}; step s uses words 3s..3s+2 of blocks 1, 2, ...  Steps run in
-//     groups of four that consume exactly three Philox blocks, so the word a
-//     step reads is fixed at compile time and lives in registers.
+//     (block, photon_lo, photon_hi, stream).  Each block yields six 21-bit
+//     uniforms; block 0 holds the emission draws {λ, φ, θ} and step 0, block
+//     b >= 1 holds steps 2b-1 and 2b.  Steps run in pairs that consume one
+//     block, so the bits a step reads are fixed at compile time.
 //   * Medium tables {n, λ_sca, λ_abs} and the spectrum knots sit in LDS.
 //   * Hits: wavefront ballot + mbcnt rank, one global atomic per wave that has
 //     any hit, 64-byte records written with 16-byte stores.
@@ -123,6 +123,10 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+__device__ __forceinline__ uint64_t pack2(float a, float b) {
+    return (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
+}
+
 struct TileCtx {
     const float4* table;  // LDS copy of the medium table
     const float* knots;   // LDS copy of the spectrum knots
@@ -137,6 +141,7 @@ struct TileCtx {
 // tile's counts and FP64 sums in a fixed order.  The result is valid in
 // thread 0 only.  Shared by the batch kernel and the serving kernel, so a
 // request's tile partials are the same whichever kernel ran them.
+template <bool kSysStores>
 __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& R,
                                                 uint32_t req_index, uint64_t first,
                                                 uint32_t count, pp_hit* hits, uint64_t hits_cap,
@@ -160,8 +165,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         const uint32_t id_lo = (uint32_t)pid, id_hi = (uint32_t)(pid >> 32);
 
         // ---- emission (src/photon_prop_cuda.jl:308-316) ----
+        // Draws: block 0 = {λ, φ, θ, step 0}; block b >= 1 = {step 2b-1, step 2b}.
         const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-        const float u_l = ppd::u01(w0.x);
+        const float u_l = ppd::u21(ppd::v21_hi(w0.x));
         float lambda;
         if (T.spec.kind == PP_SPEC_MONO) {
             lambda = T.spec.mono;
@@ -177,10 +183,10 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
 
         float dx, dy, dz;
         {
-            const float u_phi = ppd::u01(w0.y);
+            const float u_phi = ppd::u21(ppd::v21_hi(w0.y));
             const float sp = __builtin_amdgcn_sinf(u_phi);
             const float cp = __builtin_amdgcn_cosf(u_phi);
-            const float u_th = ppd::u01(w0.z);
+            const float u_th = ppd::u21(ppd::v21_hi(w0.z));
             if (R.profile == PP_EMIT_REF_ISOTROPIC) {
                 const float theta = __builtin_fmaf(u_th, 2.0f, -1.0f);
                 const float st = __builtin_amdgcn_sinf(theta * kInv2Pi);
@@ -207,7 +213,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
 #define PP_STEP(WA, WB, WC)                                                              \
     if (alive) {                                                                        \
-        const float u1 = ppd::u01(WA);                                                  \
+        const float u1 = ppd::u21(WA);                                                  \
         float s = -__builtin_amdgcn_logf(u1) * sca_ln2;                                 \
         const float rx = px - cx, ry = py - cy, rz = pz - cz;                           \
         const float a = __builtin_fmaf(dz, rz, __builtin_fmaf(dy, ry, dx * rx));        \
@@ -216,7 +222,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         ++nsteps;                                                                       \
         bool h = false;                                                                 \
         if (b >= 0.0f) {                                                                \
-            const float d = -a - sqrtf(b);                                              \
+            const float d = -a - __builtin_amdgcn_sqrtf(b);                             \
             if (d > 0.0f && d < s) {                                                    \
                 h = true;                                                               \
                 s = d;                                                                  \
@@ -230,30 +236,23 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             hit = true;                                                                 \
             alive = false;                                                              \
         } else {                                                                        \
-            const float ct = ppd::hg_costheta(T.hg, ppd::u01(WB));                      \
+            const float ct = ppd::hg_costheta(T.hg, ppd::u21(WB));                      \
             const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));     \
-            const float uph = ppd::u01(WC);                                             \
+            const float uph = ppd::u21(WC);                                             \
             const float sph = __builtin_amdgcn_sinf(uph);                               \
             const float cph = __builtin_amdgcn_cosf(uph);                               \
             ppd::rotate_to_axis(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);        \
         }                                                                               \
     }
 
-        for (uint32_t s0 = 0; s0 < max_steps; s0 += 4) {
+        PP_STEP(ppd::v21_hi(w0.w), ppd::v21_lo(w0.x, w0.y), ppd::v21_lo(w0.z, w0.w))
+        for (uint32_t s0 = 1; s0 < max_steps; s0 += 2) {
             if (__ballot(alive) == 0) break;
-            const uint32_t blk = 1u + 3u * (s0 >> 2);
-            const ppd::U4 A = ppd::philox10(blk, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(A.x, A.y, A.z)
+            const ppd::U4 Q =
+                ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(ppd::v21_hi(Q.x), ppd::v21_hi(Q.y), ppd::v21_hi(Q.z))
             if (s0 + 1 >= max_steps || __ballot(alive) == 0) break;
-            const ppd::U4 B =
-                ppd::philox10(blk + 1u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(A.w, B.x, B.y)
-            if (s0 + 2 >= max_steps || __ballot(alive) == 0) break;
-            const ppd::U4 C =
-                ppd::philox10(blk + 2u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(B.z, B.w, C.x)
-            if (s0 + 3 >= max_steps) break;
-            PP_STEP(C.y, C.z, C.w)
+            PP_STEP(ppd::v21_hi(Q.w), ppd::v21_lo(Q.x, Q.y), ppd::v21_lo(Q.z, Q.w))
         }
 #undef PP_STEP
         acc_steps += nsteps;
@@ -276,16 +275,31 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                     (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                 const unsigned long long slot = base + rank;
                 if (slot < hits_cap) {
-                    float4* o = reinterpret_cast<float4*>(hits + slot);
-                    o[0] = make_float4(px, py, pz, t);
-                    o[1] = make_float4(dx, dy, dz, lambda);
-                    o[2] = make_float4(idx, idy, idz, wgt);
-                    uint4 tail;
-                    tail.x = __float_as_uint(dist);
-                    tail.y = req_index;
-                    tail.z = id_lo;
-                    tail.w = id_hi;
-                    reinterpret_cast<uint4*>(o)[3] = tail;
+                    if (kSysStores) {
+                        // serving: the host copies records while the kernel runs,
+                        // so they go straight to memory (no L2 write-back needed)
+                        uint64_t* o = reinterpret_cast<uint64_t*>(hits + slot);
+                        const uint64_t v[8] = {
+                            pack2(px, py), pack2(pz, t), pack2(dx, dy), pack2(dz, lambda),
+                            pack2(idx, idy), pack2(idz, wgt),
+                            (uint64_t)__float_as_uint(dist) | ((uint64_t)req_index << 32),
+                            (uint64_t)id_lo | ((uint64_t)id_hi << 32)};
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            __hip_atomic_store(o + q, v[q], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
+                    } else {
+                        float4* o = reinterpret_cast<float4*>(hits + slot);
+                        o[0] = make_float4(px, py, pz, t);
+                        o[1] = make_float4(dx, dy, dz, lambda);
+                        o[2] = make_float4(idx, idy, idz, wgt);
+                        uint4 tail;
+                        tail.x = __float_as_uint(dist);
+                        tail.y = req_index;
+                        tail.z = id_lo;
+                        tail.w = id_hi;
+                        reinterpret_cast<uint4*>(o)[3] = tail;
+                    }
                 }
             }
         }
@@ -368,21 +382,36 @@ __global__ void __launch_bounds__(kBlock) propagate_kernel(KParams P) {
         const uint64_t first = ((uint64_t)tile - P.tile_start[r]) * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
-        const TilePartial tp = run_tile(T, R, r, first, count, P.hits, P.hits_cap, P.hit_cursor,
-                                        s_red, s_redi);
+        const TilePartial tp = run_tile<false>(T, R, r, first, count, P.hits, P.hits_cap,
+                                               P.hit_cursor, s_red, s_redi);
         if (threadIdx.x == 0) P.partials[tile] = tp;
     }
 }
 
 // Sum of a request's tile partials in tile order: lane l adds tiles l, l+64, ...
 // then a fixed butterfly.  Valid in every lane of the calling wave.
+template <bool kSysLoads>
 __device__ __forceinline__ pp_summary reduce_partials(const TilePartial* parts, uint64_t n_tiles,
                                                       uint64_t n_sim) {
     const uint32_t lane = threadIdx.x & 63u;
     uint64_t st = 0, nh = 0;
     double sw = 0.0, swt = 0.0, swt2 = 0.0;
     for (uint64_t t = lane; t < n_tiles; t += 64) {
-        const TilePartial p = parts[t];
+        TilePartial p;
+        if (kSysLoads) {  // written by blocks on other XCDs with system-scope stores
+            const uint64_t* q = reinterpret_cast<const uint64_t*>(parts + t);
+            uint64_t w[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                w[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            p.n_steps = w[0];
+            p.n_hits = w[1];
+            p.sum_w = __longlong_as_double((long long)w[2]);
+            p.sum_wt = __longlong_as_double((long long)w[3]);
+            p.sum_wt2 = __longlong_as_double((long long)w[4]);
+        } else {
+            p = parts[t];
+        }
         st += p.n_steps;
         nh += p.n_hits;
         sw += p.sum_w;
@@ -418,7 +447,7 @@ __global__ void __launch_bounds__(64) reduce_kernel(const TilePartial* partials,
     const uint32_t r = blockIdx.x;
     if (r >= n_req) return;
     const uint64_t b = tile_start[r], e = tile_start[r + 1];
-    const pp_summary s = reduce_partials(partials + b, e - b, reqs[r].n_photons);
+    const pp_summary s = reduce_partials<false>(partials + b, e - b, reqs[r].n_photons);
     if (threadIdx.x == 0) out[r] = s;
 }
 
@@ -1198,11 +1227,6 @@ __device__ __forceinline__ T sys_load(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-template <typename T>
-__device__ __forceinline__ T sys_load_acq(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
 __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
     extern __shared__ float4 s_table[];
     __shared__ unsigned long long s_tile;
@@ -1223,7 +1247,9 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
         if (tid == 0) {
             unsigned long long got = kSrvExit;
             for (;;) {
-                const uint64_t pub = sys_load_acq(&P.ctl->published);
+                // relaxed: every read of host-written data below is itself an
+                // uncached system-scope load, so no cache invalidation is needed
+                const uint64_t pub = sys_load(&P.ctl->published);
                 const unsigned long long t = __hip_atomic_load(
                     P.next_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (t < pub) {
@@ -1281,19 +1307,26 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
         const uint64_t hit_cap = sd->hit_cap;
         const uint64_t seq = sd->seq;
         TilePartial* parts = P.partials + (uint64_t)slot * P.max_tiles;
-        const TilePartial tp = run_tile(T, R, slot, first, count, P.pool + sd->hit_base, hit_cap,
-                                        &P.hit_cursor[slot], s_red, s_redi);
-        __syncthreads();  // every lane's hit stores issued before the fence below
+        const TilePartial tp = run_tile<true>(T, R, slot, first, count, P.pool + sd->hit_base,
+                                              hit_cap, &P.hit_cursor[slot], s_red, s_redi);
+        // every lane's (system-scope) hit stores have completed past the barrier
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
         if (tid == 0) {
-            parts[local] = tp;
-            __threadfence_system();  // partial and hit records visible to other XCDs and DMA
+            uint64_t* q = reinterpret_cast<uint64_t*>(parts + local);
+            const uint64_t w[5] = {tp.n_steps, tp.n_hits, (uint64_t)__double_as_longlong(tp.sum_w),
+                                   (uint64_t)__double_as_longlong(tp.sum_wt),
+                                   (uint64_t)__double_as_longlong(tp.sum_wt2)};
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                __hip_atomic_store(q + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // stores done before the count
             const uint32_t done = atomicAdd(&P.tiles_done[slot], 1u) + 1u;
             s_last = done == n_tiles;
-            __threadfence();
         }
         __syncthreads();
         if (s_last && tid < 64) {
-            const pp_summary s = reduce_partials(parts, n_tiles, R.n_photons);
+            const pp_summary s = reduce_partials<true>(parts, n_tiles, R.n_photons);
             if (tid == 0) {
                 const unsigned long long cur = __hip_atomic_load(
                     &P.hit_cursor[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1304,10 +1337,12 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
                 for (int k = 0; k < 8; ++k)
                     __hip_atomic_store(&P.done[slot].words[k], w[k], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
-                P.tiles_done[slot] = 0;
-                P.hit_cursor[slot] = 0;
-                __threadfence_system();
-                __hip_atomic_store(&P.done[slot].seq, seq, __ATOMIC_RELEASE,
+                __hip_atomic_store(&P.tiles_done[slot], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&P.hit_cursor[slot], 0ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __hip_atomic_store(&P.done[slot].seq, seq, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }

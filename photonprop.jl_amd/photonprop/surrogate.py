@@ -232,6 +232,13 @@ def train(mlp, ts, epochs=8, batch=4096, lr=2e-3, time_weight=0.5, seed=0, log=N
     import torch
 
     dev = "cuda:%d" % 0
+    # start the log-yield output at the data's mean yield (the output layer is
+    # initialised small, so y0 would otherwise start at log(1) and take
+    # thousands of Adam steps to reach yields of 1e-4..1e-8)
+    p = mlp.get_params()
+    p[mlp.n_params - mlp.n_out] = math.log(max(float(ts.k.sum()), 1e-30) /
+                                           float(np.exp(ts.log_n.astype(np.float64)).sum()))
+    mlp.set_params(p)
     X = torch.from_numpy(ts.x).to(dev)
     LN = torch.from_numpy(ts.log_n).to(dev)
     K = torch.from_numpy(ts.k).to(dev)

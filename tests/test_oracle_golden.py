@@ -47,6 +47,25 @@ def test_uniform_range_and_ends():
 
 # --------------------------------------------------------------- uniform ----
 
+def test_six_uniforms_range_uniformity_and_independence():
+    """The propagation draws six 21-bit uniforms per Philox block: each must be
+    on (0, 1] with the 2^-21 lattice, uniform (KS), and the two assembled from
+    low bits must not correlate with the four from high bits."""
+    blocks = [oracle.philox4x32_10([i, 7, 0, 3], [11, 0]) for i in range(20000)]
+    u = np.array([oracle.six_uniforms(b) for b in blocks], dtype=np.float64)
+    assert u.min() > 0.0 and u.max() <= 1.0
+    assert np.all(np.abs(u * 2 ** 21 - np.round(u * 2 ** 21)) == 0)
+    for j in range(6):
+        assert stats.kstest(u[:, j], "uniform").pvalue > 0.001, j
+    c = np.corrcoef(u.T)
+    assert np.abs(c - np.eye(6)).max() < 0.04
+    # all-zero and all-one words hit the lattice ends
+    np.testing.assert_array_equal(oracle.six_uniforms([0, 0, 0, 0]), [2.0 ** -21] * 6)
+    np.testing.assert_array_equal(oracle.six_uniforms([0xFFFFFFFF] * 4), [1.0] * 6)
+    # v4 takes the low 11 bits of word 0 and bits 0..9 of word 1
+    np.testing.assert_array_equal(oracle.six_uniforms([0x7FF, 0x3FF, 0, 0])[4], 1.0)
+    assert oracle.six_uniforms([0, 0x400, 0, 0])[4] == 2.0 ** -21  # bit 10 of word 1 unused
+
 def test_uniform_range_reference_kat():
     # src/tests/test_photon_prob.jl:13-17: uniform(2, 5) stays in [2, 5].
     words = [oracle.philox4x32_10([i, 0, 0, 0], [0, 0]) for i in range(2500)]
@@ -226,7 +245,8 @@ def test_wavelength_inverse_square_closed_form():
     t, sp, em, keep = _structs(m, pp.InverseSquareSpectrum(300, 600), "CherenkovCone")
     out = oracle.emit(t, sp, em, 0, 0, np.arange(4000), precision=64)
     lam = out[:, 0]
-    u = np.array([oracle.uniform(int(oracle.philox4x32_10([0, i, 0, 0], [0, 0])[0])) for i in range(4000)])
+    u = np.array([oracle.six_uniforms(oracle.philox4x32_10([0, i, 0, 0], [0, 0]))[0]
+                  for i in range(4000)], dtype=np.float64)
     np.testing.assert_allclose(lam, 600.0 / (2.0 - u), rtol=1e-12)
     # KS against the closed-form CDF, and the mean 600·ln 2
     cdf = pp.InverseSquareSpectrum(300, 600).cdf
