@@ -5,7 +5,7 @@ Medium, Spectral (spectrum), Emission, Detection and the propagation entry
 points.  Compute runs in libphotonprop.so (hand-written gfx950 HIP kernels);
 this package only converts inputs and outputs.
 """
-from . import _abi, detection, emission, medium, modelling, spectrum, utils  # noqa: F401
+from . import _abi, detection, emission, lightyield, medium, modelling, spectrum, utils  # noqa: F401
 from ._abi import HIT_DTYPE, PhotonPropError, load_library  # noqa: F401
 from .detection import DetectionSphere  # noqa: F401
 from .emission import AngularEmissionProfile, PhotonSource  # noqa: F401
