@@ -23,17 +23,23 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c in one gfx950 v_bitop3_b32 (truth table 0x96 over the operand
+// bit patterns 0xF0, 0xCC, 0xAA); plain XORs would take two instructions.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Philox4x32-10.  The key (k0, k1) is wave-uniform, so its schedule lives in
-// scalar registers; each round costs two 32x32->64 multiplies and two 3-input
-// XORs per lane.
+// scalar registers; each round costs two 32x32->64 multiplies (v_mad_u64_u32)
+// and two 3-input XORs per lane.
 __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                        uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)kM0 * c0;
         const uint64_t p1 = (uint64_t)kM1 * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -84,7 +90,10 @@ __device__ __forceinline__ float hg_costheta(const HG& h, float eta) {
 //     x' = nx·z + x·nz - y·q,  y' = ny·z + y·nz + x·q,  z' = nz·z - (x·nx + y·ny).
 // For z < 0, 1+z is taken as (x²+y²)/(1-z), which stays consistent with x, y
 // near the south pole.  At the exact pole (x = y = 0, z < 0) the result is the
-// reference's (nx, -ny, -nz).
+// reference's (nx, -ny, -nz).  Unit inputs give |r|² = 1 + e with |e| of a few
+// 1e-6 (rounding and the hardware sin/cos), so the renormalising 1/sqrt(|r|²)
+// is one Newton step from 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an
+// FP32 ulp; it needs no transcendental.
 __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float nx, float ny,
                                                float nz, float& ox, float& oy, float& oz) {
     const float s2 = __builtin_fmaf(y, y, x * x);
@@ -102,7 +111,8 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float 
     rx = pole ? nx : rx;
     ry = pole ? -ny : ry;
     rz = pole ? -nz : rz;
-    const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
+    const float n2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+    const float inv = __builtin_fmaf(-0.5f, n2, 1.5f);
     ox = rx * inv;
     oy = ry * inv;
     oz = rz * inv;

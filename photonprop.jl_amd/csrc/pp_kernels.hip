@@ -155,7 +155,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     const float r2 = R.r2;
     const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
 
-    uint32_t acc_steps = 0, acc_hits = 0;
+    uint32_t wave_steps = 0, acc_hits = 0;  // wave_steps is wave-uniform
     double acc_w = 0.0, acc_wt = 0.0, acc_wt2 = 0.0;
 
     for (uint32_t k = 0; k < rounds; ++k) {
@@ -204,22 +204,21 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             }
         }
         const float idx = dx, idy = dy, idz = dz;
-        float px = R.pos[0], py = R.pos[1], pz = R.pos[2];
+        // position relative to the target centre: the sphere test needs no
+        // per-step subtraction; the absolute position is rebuilt for a hit
+        float qx = R.pos[0] - cx, qy = R.pos[1] - cy, qz = R.pos[2] - cz;
         float dist = 0.0f;
         bool alive = active;
         bool hit = false;
-        uint32_t nsteps = 0;
 
         // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
 #define PP_STEP(WA, WB, WC)                                                              \
     if (alive) {                                                                        \
         const float u1 = ppd::u21(WA);                                                  \
         float s = -__builtin_amdgcn_logf(u1) * sca_ln2;                                 \
-        const float rx = px - cx, ry = py - cy, rz = pz - cz;                           \
-        const float a = __builtin_fmaf(dz, rz, __builtin_fmaf(dy, ry, dx * rx));        \
-        const float pp = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));       \
+        const float a = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));        \
+        const float pp = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));       \
         const float b = __builtin_fmaf(a, a, r2 - pp);                                  \
-        ++nsteps;                                                                       \
         bool h = false;                                                                 \
         if (b >= 0.0f) {                                                                \
             const float d = -a - __builtin_amdgcn_sqrtf(b);                             \
@@ -228,9 +227,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                 s = d;                                                                  \
             }                                                                           \
         }                                                                               \
-        px = __builtin_fmaf(s, dx, px);                                                 \
-        py = __builtin_fmaf(s, dy, py);                                                 \
-        pz = __builtin_fmaf(s, dz, pz);                                                 \
+        qx = __builtin_fmaf(s, dx, qx);                                                 \
+        qy = __builtin_fmaf(s, dy, qy);                                                 \
+        qz = __builtin_fmaf(s, dz, qz);                                                 \
         dist += s;                                                                      \
         if (h) {                                                                        \
             hit = true;                                                                 \
@@ -245,17 +244,24 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         }                                                                               \
     }
 
+        // Steps are counted per wave from the alive ballot (scalar unit).
+        uint64_t am = __ballot(alive);
+        wave_steps += (uint32_t)__popcll(am);
         PP_STEP(ppd::v21_hi(w0.w), ppd::v21_lo(w0.x, w0.y), ppd::v21_lo(w0.z, w0.w))
         for (uint32_t s0 = 1; s0 < max_steps; s0 += 2) {
-            if (__ballot(alive) == 0) break;
+            am = __ballot(alive);
+            if (am == 0) break;
+            wave_steps += (uint32_t)__popcll(am);
             const ppd::U4 Q =
                 ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
             PP_STEP(ppd::v21_hi(Q.x), ppd::v21_hi(Q.y), ppd::v21_hi(Q.z))
-            if (s0 + 1 >= max_steps || __ballot(alive) == 0) break;
+            if (s0 + 1 >= max_steps) break;
+            am = __ballot(alive);
+            if (am == 0) break;
+            wave_steps += (uint32_t)__popcll(am);
             PP_STEP(ppd::v21_hi(Q.w), ppd::v21_lo(Q.x, Q.y), ppd::v21_lo(Q.z, Q.w))
         }
 #undef PP_STEP
-        acc_steps += nsteps;
 
         // ---- hit append: one atomic per wave with any hit ----
         const uint64_t mask = __ballot(hit);
@@ -264,6 +270,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             if (lane == 0) base = atomicAdd(hit_cursor, (unsigned long long)__popcll(mask));
             base = __shfl(base, 0, 64);
             if (hit) {
+                const float px = qx + cx, py = qy + cy, pz = qz + cz;
                 const float e = -dist / abs_len;
                 const float wgt = __builtin_amdgcn_exp2f(e * kLog2e);
                 const float t = R.t0 + dist * n_phase / (float)kCVacuum;
@@ -306,7 +313,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     }
 
     // ---- tile partial: fixed-order reduction ----
-    const uint32_t ws = wave_sum_u32(acc_steps);
+    const uint32_t ws = wave_steps;
     const uint32_t wh = wave_sum_u32(acc_hits);
     const double w1 = wave_sum_f64(acc_w);
     const double w2 = wave_sum_f64(acc_wt);
@@ -357,7 +364,7 @@ __device__ __forceinline__ TileCtx stage_tables(float4* s_table, const float4* t
     return T;
 }
 
-__global__ void __launch_bounds__(kBlock) propagate_kernel(KParams P) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) propagate_kernel(KParams P) {
     extern __shared__ float4 s_table[];  // n_points float4, then knots
     __shared__ uint32_t s_tile;
     __shared__ double s_red[kWaves][3];
@@ -1160,18 +1167,26 @@ int pp_selftest_hg_moments(pp_ctx* c, float g, uint64_t seed, uint64_t n, double
 // hits_cap records of the device hit pool are free -> the request's slot
 // descriptor and a range entry [t0, t1) of the global tile sequence are
 // written to pinned host memory -> `published` advances past t1.
-// Device side (persistent kernel, one resident block per CU slot): a block
-// claims the next global tile with a CAS on a device counter (only while the
-// tile is below `published`), finds its range entry by a 64-ary search, runs
-// the tile with the same run_tile as the batch kernel, stores the tile
-// partial, and the block that completes a request's last tile adds the
-// partials in tile order and posts the summary to pinned memory.  Blocks
-// leave when the host sets `stop` or after idle_exit_us without work.
+// Device side (persistent kernel, every resident block slot filled): a block
+// takes a ticket (the next global tile index) with one atomicAdd and waits
+// until the tile is published.  The published count is mirrored from pinned
+// memory into a device-memory gate word; only the block holding the lowest
+// unpublished ticket (the head) reads the host's counters over PCIe, so the
+// link carries one poller however many blocks wait.  A block then finds its
+// range entry by a 64-ary search, runs the tile with the same run_tile as the
+// batch kernel, stores the tile partial, and the block that completes a
+// request's last tile adds the partials in tile order and posts the summary
+// to pinned memory.  The host stops the kernel (`stop`, raised into the gate
+// by the head) only when every published tile is done, so the tickets held at
+// exit are all unpublished and the next launch restarts the ticket counter at
+// the published count.
 
 namespace {
 
 constexpr uint32_t kSrvRing = 16384;          // range-entry ring
 constexpr uint64_t kSrvExit = ~0ull;
+constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
+constexpr size_t kSrvGateOffset = 4096;  // ticket and gate words on separate pages
 
 struct SrvEntry {
     uint64_t t0, t1;
@@ -1206,7 +1221,8 @@ struct SrvParams {
     const SrvEntry* entries;
     const SrvSlot* slots;
     SrvDone* done;
-    unsigned long long* next_tile;
+    unsigned long long* ticket;       // next ticket (global tile index)
+    unsigned long long* gate;         // published tiles mirrored from ctl | kGateStop
     uint32_t* tiles_done;             // [max_slots]
     unsigned long long* hit_cursor;   // [max_slots]
     TilePartial* partials;            // [max_slots][max_tiles]
@@ -1218,8 +1234,6 @@ struct SrvParams {
     float lmin, inv_dl;
     ppd::HG hg;
     SpecParams spec;
-    uint64_t idle_ticks;              // s_memrealtime ticks (100 MHz)
-    uint64_t life_ticks;
 };
 
 template <typename T>
@@ -1239,31 +1253,42 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
     const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
                                    P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
-    const uint64_t born = __builtin_amdgcn_s_memrealtime();
-    uint64_t idle_since = born;
 
     for (;;) {
         __syncthreads();
         if (tid == 0) {
+            const unsigned long long t = atomicAdd(P.ticket, 1ull);
             unsigned long long got = kSrvExit;
             for (;;) {
-                // relaxed: every read of host-written data below is itself an
-                // uncached system-scope load, so no cache invalidation is needed
-                const uint64_t pub = sys_load(&P.ctl->published);
-                const unsigned long long t = __hip_atomic_load(
-                    P.next_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (t < pub) {
-                    if (atomicCAS(P.next_tile, t, t + 1) == t) {
+                // agent-scope atomic load (sc1): coherent across the XCDs' L2s
+                const unsigned long long g =
+                    __hip_atomic_load(P.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t < (g & ~kGateStop)) {
+                    got = t;
+                    break;
+                }
+                if (g & kGateStop) break;
+                if (t == g) {
+                    // head of the queue: the one block that reads pinned memory.
+                    // Relaxed is enough: every later read of host-written data
+                    // is itself an uncached system-scope load.
+                    const uint64_t hp = sys_load(&P.ctl->published);
+                    if (hp > t) {
+                        atomicMax(P.gate, (unsigned long long)hp);
                         got = t;
                         break;
                     }
-                    continue;
+                    if (sys_load(&P.ctl->stop)) {
+                        atomicOr(P.gate, kGateStop);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(16);
+                } else {
+                    // the farther behind the head, the longer between polls
+                    const uint64_t behind = t - g;
+                    const uint32_t reps = behind >= 448 ? 8u : 1u + (uint32_t)(behind >> 6);
+                    for (uint32_t k = 0; k < reps; ++k) __builtin_amdgcn_s_sleep(127);
                 }
-                const uint64_t now = __builtin_amdgcn_s_memrealtime();
-                if (sys_load(&P.ctl->stop) || now - idle_since > P.idle_ticks ||
-                    now - born > P.life_ticks)
-                    break;
-                __builtin_amdgcn_s_sleep(64);
             }
             s_tile = got;
         }
@@ -1337,16 +1362,22 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
                 for (int k = 0; k < 8; ++k)
                     __hip_atomic_store(&P.done[slot].words[k], w[k], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
-                __hip_atomic_store(&P.tiles_done[slot], 0u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&P.hit_cursor[slot], 0ull, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                atomicExch(&P.tiles_done[slot], 0u);
+                atomicExch(&P.hit_cursor[slot], 0ull);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __hip_atomic_store(&P.done[slot].seq, seq, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
-        idle_since = __builtin_amdgcn_s_memrealtime();
+    }
+}
+
+// Restart the ticket counter and the gate at the first unprocessed tile.
+__global__ void serve_reset_kernel(unsigned long long* ticket, unsigned long long* gate,
+                                   unsigned long long first) {
+    if (threadIdx.x == 0) {
+        atomicExch(ticket, first);
+        atomicExch(gate, first);
     }
 }
 
@@ -1376,6 +1407,9 @@ struct pp_server {
     hipStream_t kstream = nullptr, cstream = nullptr;
     hipEvent_t kdone = nullptr;
     bool kernel_live = false;
+    bool stopping = false;       // stop raised, kernel not yet finished: admit nothing
+    uint64_t resume_tile = 0;    // tiles processed when the kernel last stopped
+    std::chrono::steady_clock::time_point idle_since;
     uint64_t launches = 0;
     // pinned
     SrvControl* ctl = nullptr;
@@ -1383,7 +1417,7 @@ struct pp_server {
     SrvSlot* slots = nullptr;
     SrvDone* done = nullptr;
     // device
-    DevBuf table, knots, next_tile, tiles_done, hit_cursor, partials, pool;
+    DevBuf table, knots, dctl, tiles_done, hit_cursor, partials, pool;
     // host bookkeeping (guarded by mu)
     std::mutex mu;
     std::condition_variable cv;
@@ -1447,7 +1481,8 @@ int launch_serve_kernel(pp_server* s) {
     P.entries = s->entries;
     P.slots = s->slots;
     P.done = s->done;
-    P.next_tile = (unsigned long long*)s->next_tile.p;
+    P.ticket = (unsigned long long*)s->dctl.p;
+    P.gate = (unsigned long long*)((char*)s->dctl.p + kSrvGateOffset);
     P.tiles_done = (uint32_t*)s->tiles_done.p;
     P.hit_cursor = (unsigned long long*)s->hit_cursor.p;
     P.partials = (TilePartial*)s->partials.p;
@@ -1460,8 +1495,8 @@ int launch_serve_kernel(pp_server* s) {
     P.inv_dl = s->inv_dl;
     P.hg = make_hg(s->g);
     P.spec = s->spec;
-    P.idle_ticks = (uint64_t)s->cfg.idle_exit_us * 100;  // 100 MHz
-    P.life_ticks = 30ull * 100000000ull;                  // relaunched by the worker
+    // every tile below resume_tile is done and none above it was started
+    serve_reset_kernel<<<1, 64, 0, s->kstream>>>(P.ticket, P.gate, s->resume_tile);
     serve_kernel<<<s->grid, kBlock, s->lds, s->kstream>>>(P);
     PP_HIP(hipGetLastError());
     PP_HIP(hipEventRecord(s->kdone, s->kstream));
@@ -1472,6 +1507,7 @@ int launch_serve_kernel(pp_server* s) {
 
 // Admit pending jobs (FIFO) while a slot, pool room and ring room exist.
 void admit_locked(pp_server* s) {
+    if (s->stopping) return;  // tickets restart at resume_tile on the next launch
     bool published_any = false;
     while (!s->pending.empty() && !s->free_slots.empty() &&
            s->n_entries - s->first_live < kSrvRing) {
@@ -1557,9 +1593,26 @@ void worker_main(pp_server* s) {
         const bool any = reap_locked(s);
         admit_locked(s);
         if (any) s->cv.notify_all();
+        if (s->kernel_live && hipEventQuery(s->kdone) == hipSuccess) {
+            s->kernel_live = false;
+            if (s->stopping) {
+                s->stopping = false;
+                __atomic_store_n(&s->ctl->stop, 0ull, __ATOMIC_RELEASE);
+                admit_locked(s);
+            }
+        }
         const bool busy = s->completed < s->submitted;
-        if (s->kernel_live && hipEventQuery(s->kdone) == hipSuccess) s->kernel_live = false;
-        if (busy && !s->kernel_live && s->published > 0) {
+        const auto now = std::chrono::steady_clock::now();
+        if (busy) s->idle_since = now;
+        if (s->kernel_live && !s->stopping && !busy &&
+            now - s->idle_since > std::chrono::microseconds(s->cfg.idle_exit_us)) {
+            // every published tile is done: the blocks all hold unpublished
+            // tickets, and the next launch resumes from here
+            s->stopping = true;
+            s->resume_tile = s->published;
+            __atomic_store_n(&s->ctl->stop, 1ull, __ATOMIC_RELEASE);
+        }
+        if (busy && !s->kernel_live && s->published > s->resume_tile) {
             if (launch_serve_kernel(s) != PP_OK) {
                 s->worker_error = last_error();
                 s->cv.notify_all();
@@ -1581,7 +1634,7 @@ void free_server(pp_server* s) {
     if (s->entries) (void)hipHostFree(s->entries);
     if (s->slots) (void)hipHostFree(s->slots);
     if (s->done) (void)hipHostFree(s->done);
-    DevBuf* bufs[] = {&s->table, &s->knots, &s->next_tile, &s->tiles_done, &s->hit_cursor,
+    DevBuf* bufs[] = {&s->table, &s->knots, &s->dctl, &s->tiles_done, &s->hit_cursor,
                       &s->partials, &s->pool};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1622,14 +1675,14 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     memset(s->entries, 0, kSrvRing * sizeof(SrvEntry));
     memset(s->slots, 0, S * sizeof(SrvSlot));
     memset(s->done, 0, S * sizeof(SrvDone));
-    if ((rc = ensure(s->next_tile, 64))) return rc;
+    if ((rc = ensure(s->dctl, 2 * kSrvGateOffset))) return rc;
     if ((rc = ensure(s->tiles_done, S * sizeof(uint32_t)))) return rc;
     if ((rc = ensure(s->hit_cursor, S * sizeof(unsigned long long)))) return rc;
     if ((rc = ensure(s->partials, (size_t)S * s->cfg.max_tiles_per_request * sizeof(TilePartial))))
         return rc;
     if ((rc = ensure(s->pool, std::max<uint64_t>(s->cfg.hit_pool_records, 1) * sizeof(pp_hit))))
         return rc;
-    fill_u32_kernel<<<1, 64, 0, s->kstream>>>((uint32_t*)s->next_tile.p, 16, 0u);
+    fill_u32_kernel<<<8, 256, 0, s->kstream>>>((uint32_t*)s->dctl.p, 2 * kSrvGateOffset / 4, 0u);
     fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->tiles_done.p, S, 0u);
     fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->hit_cursor.p, 2ull * S, 0u);
     PP_HIP(hipGetLastError());

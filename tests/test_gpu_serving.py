@@ -116,3 +116,29 @@ def test_server_open_loop_poisson(ctx):
            st["kernel_launches"]))
     assert np.all(lat > 0)
     assert out["steps"] > 9 * out["photons"]
+
+
+def test_server_idle_stop_and_relaunch(ctx):
+    """The kernel stops after idle_exit_us without work and is relaunched on
+    the next submission; tickets resume at the published count, so results
+    after the relaunch are still the same as alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(6, seed=33)
+    reqs["n_photons"] = [200_000, 50_000, 1_000_000, 8192, 8193, 300_000]
+    with pp.Server(ctx, spec, max_slots=4, max_tiles_per_request=256, hit_pool_records=1 << 20,
+                   idle_exit_us=2000) as srv:
+        got = []
+        for wave in (reqs[:3], reqs[3:]):
+            ts = [srv.submit(r, int(r["n_photons"])) for r in wave]
+            for t, r in zip(ts, wave):
+                assert srv.wait(t, 60000)
+                got.append(srv.result(t, int(r["n_photons"])))
+            time.sleep(0.05)  # well past idle_exit_us: the kernel stops
+        st = srv.stats()
+    say("idle/relaunch stats", st)
+    assert st["kernel_launches"] >= 2
+    for r, (s, h, _) in zip(reqs, got):
+        ref = alone(ctx, r)
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (k, s[k], ref.summary[k])
+        np.testing.assert_array_equal(key(h), key(ref.hits))
