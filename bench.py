@@ -28,6 +28,12 @@ import numpy as np  # noqa: E402
 FLOPS_PER_STEP = 81          # canonical FP32 flops per scattering-loop iteration
 PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 peak
 HIT_BYTES = 64
+# HBM bytes per propagate_kernel launch of the default workload (64 geometries x
+# 1e9 photons): rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors from a
+# 1 GiB device copy (profiles/propagate_kernel_64geom_1e9.md).  Reported only
+# for that workload.
+PROFILED_PHOTONS = 1e9
+PROFILED_TRAFFIC_BYTES = 1.999e10
 
 
 def workload_requests(pp, photons_per_geometry, seed=0):
@@ -110,21 +116,38 @@ def bench_batched(pp, ctx, spectrum, reps=3):
 
 
 def per_request_open_loop(pp, ctx, spectrum, arrivals, reqs):
-    """One pp_propagate launch per request, served FIFO as the requests arrive
-    (Poisson schedule): latency = completion - arrival."""
+    """One pp_propagate call per request (kernels, reduction, summary copy and
+    stream sync each time), served FIFO as the requests arrive (Poisson
+    schedule): latency = completion - arrival.  The C structs are built before
+    the schedule starts, so each request costs exactly one C-ABI call."""
+    import ctypes
+
+    from photonprop import propagation as prp
+
+    spec, keep = prp.spectrum_struct(spectrum)
+    ems, tgs = [], []
+    for r in reqs:
+        ems.append(prp.emitter_struct(pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0,
+                                                      int(r["n_photons"]), spectrum)))
+        tgs.append(prp.sphere_struct(pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))))
+    seeds = [int(x) for x in reqs["seed"]]
+    streams = [int(x) for x in reqs["stream"]]
+    summ = pp._abi.pp_summary()
+    lib, h = ctx.lib, ctx.handle
     lat = np.zeros(len(reqs))
     steps = 0
     t0 = time.perf_counter()
-    for i, r in enumerate(reqs):
+    for i in range(len(reqs)):
         now = time.perf_counter() - t0
         if now < arrivals[i]:
             time.sleep(arrivals[i] - now)
-        src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0, int(r["n_photons"]), spectrum)
-        res = ctx.propagate(src, pp.DetectionSphere(tuple(r["center"]), float(r["radius"])),
-                            seed=int(r["seed"]), stream=int(r["stream"]), hits_cap=0,
-                            allow_overflow=True)
-        steps += res.summary["n_steps"]
+        rc = lib.pp_propagate(h, ctypes.byref(ems[i]), ctypes.byref(spec), ctypes.byref(tgs[i]),
+                              seeds[i], streams[i], 10, None, 0, ctypes.byref(summ))
+        if rc not in (0, pp._abi.PP_ERR_OVERFLOW):
+            pp._abi.check(rc, lib)
+        steps += summ.n_steps
         lat[i] = 1e6 * (time.perf_counter() - t0 - arrivals[i])
+    del keep
     return {"latency_us": lat, "makespan_s": time.perf_counter() - t0,
             "photons": int(reqs["n_photons"].sum()), "steps": steps}
 
@@ -140,42 +163,61 @@ def summarize(res, n_req):
     }
 
 
-def bench_serving(pp, ctx, spectrum, n_req, rates=(500.0, 1000.0, 2000.0, 5000.0, 10000.0, 20000.0)):
-    """Poisson arrivals of requests (budgets log-uniform 1e4..1e8 photons, random
-    pose, target 1-150 m log-uniform) at several rates, served (a) by the
-    continuous-batching server and (b) by one pp_propagate launch per request,
-    FIFO.  Both report p50/p99 latency (arrival or submission to completion) and
-    delivered throughput; the headline compares the highest rate each sustains
-    with p99 < 20 ms."""
+MIXED_RATES = (500.0, 1000.0, 1500.0, 2000.0, 2200.0, 2400.0, 2600.0, 2800.0, 5000.0, 20000.0)
+SMALL_RATES = (2000.0, 5000.0, 10000.0, 20000.0, 40000.0, 80000.0)
+
+
+def _serving_sweep(pp, ctx, spectrum, n_req, rates, photons, label):
+    """Open-loop Poisson sweep of one request mix through (a) the server and
+    (b) one pp_propagate launch per request (FIFO).  Latency is measured from
+    the scheduled arrival in both.  A rate is sustained when p99 < 20 ms and at
+    least 95 % of the offered rate is delivered."""
     from photonprop import workloads
 
-    out = {"requests_per_rate": n_req, "server": {}, "per_request_launch": {}}
+    out = {"photons_per_request": list(photons), "requests_per_rate": n_req, "server": {},
+           "per_request_launch": {}}
     for rate in rates:
-        arr, reqs = workloads.serving_requests(n_req, rate, seed=11)
+        arr, reqs = workloads.serving_requests(n_req, rate, seed=11, photons=photons)
         out["per_request_launch"][str(int(rate))] = summarize(
             per_request_open_loop(pp, ctx, spectrum, arr, reqs), n_req)
-        print("per-request launch %d/s: %s" % (rate, out["per_request_launch"][str(int(rate))]),
+        print("%s per-request launch %d/s: %s" % (label, rate,
+                                                   out["per_request_launch"][str(int(rate))]),
               file=sys.stderr, flush=True)
     with pp.Server(ctx, spectrum, max_slots=1024, max_tiles_per_request=16384,
                    hit_pool_records=1 << 20) as srv:
         for rate in rates:
-            arr, reqs = workloads.serving_requests(n_req, rate, seed=11)
+            arr, reqs = workloads.serving_requests(n_req, rate, seed=11, photons=photons)
             out["server"][str(int(rate))] = summarize(pp.run_open_loop(srv, arr, reqs, hits_cap=0),
                                                      n_req)
-            print("server %d/s: %s" % (rate, out["server"][str(int(rate))]), file=sys.stderr,
-                  flush=True)
+            print("%s server %d/s: %s" % (label, rate, out["server"][str(int(rate))]),
+                  file=sys.stderr, flush=True)
         out["server_stats"] = srv.stats()
 
-    def best(d):
-        ok = [v["requests_per_s"] for v in d.values() if v["p99_ms"] < 20.0]
+    def sustained(d):
+        ok = [v["requests_per_s"] for r, v in d.items()
+              if v["p99_ms"] < 20.0 and v["requests_per_s"] >= 0.95 * float(r)]
         return max(ok) if ok else None
 
-    out["server_best_requests_per_s_at_p99_under_20ms"] = best(out["server"])
-    out["per_request_best_requests_per_s_at_p99_under_20ms"] = best(out["per_request_launch"])
-    a, b = out["server_best_requests_per_s_at_p99_under_20ms"], out[
-        "per_request_best_requests_per_s_at_p99_under_20ms"]
-    out["server_over_per_request_at_p99_under_20ms"] = (a / b) if (a and b) else None
+    a, b = sustained(out["server"]), sustained(out["per_request_launch"])
+    out["server_sustained_requests_per_s_p99_under_20ms"] = a
+    out["per_request_sustained_requests_per_s_p99_under_20ms"] = b
+    out["server_over_per_request_sustained"] = (a / b) if (a and b) else None
+    out["server_saturated_requests_per_s"] = max(v["requests_per_s"] for v in out["server"].values())
+    out["per_request_saturated_requests_per_s"] = max(
+        v["requests_per_s"] for v in out["per_request_launch"].values())
     return out
+
+
+def bench_serving(pp, ctx, spectrum, n_req):
+    """The serving workload (budgets log-uniform 1e4..1e8 photons, random
+    pose, target 1-150 m log-uniform, Poisson arrivals) and a small-request
+    mix (1e3..1e5 photons) where per-launch overhead rather than photon work
+    bounds the one-launch-per-request path."""
+    return {
+        "mixed_1e4_1e8": _serving_sweep(pp, ctx, spectrum, n_req, MIXED_RATES, (1e4, 1e8), "mixed"),
+        "small_1e3_1e5": _serving_sweep(pp, ctx, spectrum, 4 * n_req, SMALL_RATES, (1e3, 1e5),
+                                        "small"),
+    }
 
 
 def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, n_eval=512, eval_photons=20_000_000):
@@ -201,7 +243,7 @@ def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, n_eval=512, eval_p
           flush=True)
     mlp = sg.MLP(ctx, seed=0)
     t0 = time.perf_counter()
-    hist = sg.train(mlp, ts, epochs=epochs, batch=4096, lr=2e-3, time_weight=0.5,
+    hist = sg.train(mlp, ts, epochs=epochs, batch=4096, lr=3e-3, time_weight=0.5,
                     log=lambda m: print("surrogate " + m, file=sys.stderr, flush=True))
     out["train_s"] = time.perf_counter() - t0
     out["epochs"] = epochs
@@ -270,7 +312,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e9, help="photons per geometry")
-    ap.add_argument("--cpu-sample", type=float, default=1e6,
+    ap.add_argument("--cpu-sample", type=float, default=1e7,
                     help="photons per geometry for the CPU baseline (0 skips it)")
     ap.add_argument("--extras", type=int, default=1,
                     help="also measure the 4096-pair batch and the serving loop (0 skips)")
@@ -279,7 +321,7 @@ def main():
     ap.add_argument("--surrogate-rows", type=float, default=2e6,
                     help="MC training rows for the surrogate measurement (0 skips it)")
     ap.add_argument("--surrogate-photons", type=float, default=1e5, help="photons per training row")
-    ap.add_argument("--surrogate-epochs", type=int, default=10)
+    ap.add_argument("--surrogate-epochs", type=int, default=40)
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter "
                          "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -384,9 +426,12 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": PROFILED_TRAFFIC_BYTES if n_per == int(PROFILED_PHOTONS) else None,
+            "traffic_over_algorithmic": (PROFILED_TRAFFIC_BYTES / (hits_per_launch * HIT_BYTES)
+                                         if n_per == int(PROFILED_PHOTONS) else None),
             "note": "FP32 vector ALU (not MFMA): %d flops per photon step x steps per launch / "
-                    "average propagate_kernel duration (HIP events on the library's stream)"
+                    "average propagate_kernel duration (HIP events on the library's stream); "
+                    "the kernel is bound by VALU issue, HBM carries only the 64-B hit records"
                     % FLOPS_PER_STEP,
         },
     }

@@ -245,6 +245,7 @@ typedef struct pp_server_stats {
     uint64_t kernel_launches;       /* persistent-kernel launches so far */
     uint64_t pool_records_free;
     uint64_t tiles_published;
+    uint64_t grid_blocks;           /* persistent-kernel blocks (one per resident slot) */
 } pp_server_stats;
 
 typedef struct pp_server pp_server;

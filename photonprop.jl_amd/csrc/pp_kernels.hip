@@ -123,10 +123,6 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint64_t pack2(float a, float b) {
-    return (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
-}
-
 struct TileCtx {
     const float4* table;  // LDS copy of the medium table
     const float* knots;   // LDS copy of the spectrum knots
@@ -285,14 +281,16 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                     if (kSysStores) {
                         // serving: the host copies records while the kernel runs,
                         // so they go straight to memory (no L2 write-back needed)
-                        uint64_t* o = reinterpret_cast<uint64_t*>(hits + slot);
-                        const uint64_t v[8] = {
-                            pack2(px, py), pack2(pz, t), pack2(dx, dy), pack2(dz, lambda),
-                            pack2(idx, idy), pack2(idz, wgt),
-                            (uint64_t)__float_as_uint(dist) | ((uint64_t)req_index << 32),
-                            (uint64_t)id_lo | ((uint64_t)id_hi << 32)};
+                        // (32-bit words: no register pairs to assemble)
+                        uint32_t* o = reinterpret_cast<uint32_t*>(hits + slot);
+                        const uint32_t v[16] = {
+                            __float_as_uint(px),  __float_as_uint(py),  __float_as_uint(pz),
+                            __float_as_uint(t),   __float_as_uint(dx),  __float_as_uint(dy),
+                            __float_as_uint(dz),  __float_as_uint(lambda),
+                            __float_as_uint(idx), __float_as_uint(idy), __float_as_uint(idz),
+                            __float_as_uint(wgt), __float_as_uint(dist), req_index, id_lo, id_hi};
 #pragma unroll
-                        for (int q = 0; q < 8; ++q)
+                        for (int q = 0; q < 16; ++q)
                             __hip_atomic_store(o + q, v[q], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_SYSTEM);
                     } else {
@@ -1187,6 +1185,7 @@ constexpr uint32_t kSrvRing = 16384;          // range-entry ring
 constexpr uint64_t kSrvExit = ~0ull;
 constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
 constexpr size_t kSrvGateOffset = 4096;  // ticket and gate words on separate pages
+constexpr int kSrvVgprReserve = 64;      // VGPRs per SIMD lane left for other kernels
 
 struct SrvEntry {
     uint64_t t0, t1;
@@ -1689,8 +1688,18 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     PP_HIP(hipStreamSynchronize(s->kstream));
     s->lds = (size_t)s->n_points * sizeof(float4) +
              (s->spec.kind == PP_SPEC_KNOTS ? (size_t)s->spec.n_knots * sizeof(float) : 0);
+    // The persistent kernel must leave room on every CU for other kernels: the
+    // runtime's copy kernels behind pp_server_result's device-to-host copy
+    // among them.  If it filled the register file, a result copy would wait
+    // for the kernel to exit while the kernel waits (busy) for the request
+    // that the copy's pool release would admit.  Each block puts one wave on
+    // each SIMD; keep kSrvVgprReserve of the 512 VGPRs per lane free.
     int per_cu = 0;
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, s->lds));
+    hipFuncAttributes fa;
+    PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
+    const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
+    per_cu = std::min(per_cu, (512 - kSrvVgprReserve) / vgpr);
     s->grid = std::max(1, per_cu) * c->n_cu;
     s->free_slots.reserve(S);
     for (uint32_t i = 0; i < S; ++i) s->free_slots.push_back(S - 1 - i);
@@ -1820,6 +1829,7 @@ int pp_server_stats_get(pp_server* s, pp_server_stats* out) {
     out->kernel_launches = s->launches;
     out->pool_records_free = s->pool_free_total;
     out->tiles_published = s->published;
+    out->grid_blocks = (uint64_t)s->grid;
     return PP_OK;
 }
 

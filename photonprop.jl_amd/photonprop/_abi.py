@@ -113,7 +113,7 @@ class pp_server_config(ctypes.Structure):
 class pp_server_stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "completed", "in_flight", "pending", "kernel_launches",
-        "pool_records_free", "tiles_published")]
+        "pool_records_free", "tiles_published", "grid_blocks")]
 
 
 PP_TILE_PHOTONS = 8192
@@ -205,6 +205,14 @@ def load_library(path=None):
     p = path or os.environ.get("PHOTONPROP_LIB", LIB_PATH)
     if not os.path.exists(p):
         raise OSError("libphotonprop.so not found at %s; run __graft_entry__.build()" % p)
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64 (SONAME
+    # libamdhip64.so.7) and links it as "libamdhip64.so"; loaded first, it also
+    # satisfies this library's libamdhip64.so.7.  Loaded after the system
+    # runtime, it would come in as a second runtime that sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

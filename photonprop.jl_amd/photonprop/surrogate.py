@@ -226,7 +226,7 @@ class MLP:
         return loss
 
 
-def train(mlp, ts, epochs=8, batch=4096, lr=2e-3, time_weight=0.5, seed=0, log=None):
+def train(mlp, ts, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0, log=None):
     """Adam over shuffled minibatches held in device memory (torch tensors as
     plumbing); cosine learning-rate decay.  Returns per-epoch mean losses."""
     import torch
