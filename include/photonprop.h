@@ -323,7 +323,8 @@ int pp_mlp_train_step_device(pp_mlp* mlp, const float* d_x, const float* d_log_n
 int pp_selftest_rotate(pp_ctx* ctx, const float* old_dirs, const float* new_dirs, float* out,
                        uint64_t n);
 
-/* out[i] = Henyey-Greenstein cosθ for g from Philox word i of (seed, stream 0)
+/* out[i] = Henyey-Greenstein cosθ for g from Philox word i of (seed, stream 0),
+ * drawn as the propagation draws it: a = 1 + (w >> 9)·2^-23, η = 2 - a
  * (src/photon_prop_cuda.jl:31-38). */
 int pp_selftest_hg(pp_ctx* ctx, float g, uint64_t seed, float* out, uint64_t n);
 

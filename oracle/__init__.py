@@ -52,7 +52,7 @@ def lib():
         L.or_philox4x32_10.argtypes = [vp, vp, vp]
         L.or_uniform.restype = ctypes.c_float
         L.or_uniform.argtypes = [ctypes.c_uint32]
-        L.or_six_uniforms.argtypes = [vp, vp]
+        L.or_block_draws.argtypes = [vp, vp, vp]
         L.or_rodrigues_rotation.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -78,11 +78,13 @@ def uniform(word):
     return lib().or_uniform(int(word))
 
 
-def six_uniforms(words):
+def block_draws(words):
+    """(u[4] on (0, 1], p[2] azimuths in revolutions) of one Philox block."""
     w = np.ascontiguousarray(words, dtype=np.uint32)
-    out = np.zeros(6, dtype=np.float32)
-    lib().or_six_uniforms(_p(w), _p(out))
-    return out
+    u = np.zeros(4, dtype=np.float32)
+    p = np.zeros(2, dtype=np.float32)
+    lib().or_block_draws(_p(w), _p(u), _p(p))
+    return u, p
 
 
 def hg_costheta(g, eta, precision=32):

@@ -89,15 +89,15 @@ void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32
  * Used by the device self-tests of single helpers. */
 float or_uniform(uint32_t w) { return fmaf((float)(w >> 8), 0x1p-24f, 0x1p-24f); }
 
-/* Six 21-bit uniforms on (0, 1] from one Philox block (the propagation's
- * draws): v0..v3 are the top 21 bits of words 0..3; v4 joins the low 11 bits of
- * word 0 with the low 10 bits of word 1, v5 likewise words 2 and 3; each
- * u = (v + 1)·2^-21, exact in FP32. */
-void or_six_uniforms(const uint32_t w[4], float u[6]) {
-    uint32_t v[6] = {w[0] >> 11, w[1] >> 11, w[2] >> 11, w[3] >> 11,
-                     (w[0] & 0x7FFu) | ((w[1] & 0x3FFu) << 11),
-                     (w[2] & 0x7FFu) | ((w[3] & 0x3FFu) << 11)};
-    for (int i = 0; i < 6; ++i) u[i] = fmaf((float)v[i], 0x1p-21f, 0x1p-21f);
+/* The propagation's draws from one Philox block (words w0..w3):
+ *   u[k] = 1 - (w_k >> 9)·2^-23 on (0, 1], k = 0..3 (the top 23 bits; exact in
+ *          FP32; the device forms a_k = 1 + (w_k >> 9)·2^-23 and u = 2 - a_k);
+ *   p[0] = ((w0 & 0xFF) << 8 | (w1 & 0xFF))·2^-16, p[1] likewise from w2, w3:
+ *          16-bit azimuths in revolutions on [0, 1) from the low bytes. */
+void or_block_draws(const uint32_t w[4], float u[4], float p[2]) {
+    for (int k = 0; k < 4; ++k) u[k] = 1.0f - (float)(w[k] >> 9) * 0x1p-23f;
+    p[0] = (float)(((w[0] & 0xFFu) << 8) | (w[1] & 0xFFu)) * 0x1p-16f;
+    p[1] = (float)(((w[2] & 0xFFu) << 8) | (w[3] & 0xFFu)) * 0x1p-16f;
 }
 
 /* Rodrigues rotation taking a to b, applied to operand, in FP64

@@ -185,15 +185,15 @@ void FN(or_emit)(const pp_medium_tables* med, const pp_spectrum* sp, const pp_em
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t ctr[4] = {0u, (uint32_t)pid, (uint32_t)(pid >> 32), stream};
     uint32_t w0[4];
-    float u0[6];
+    float u0[4], p0[2];
     or_philox4x32_10(ctr, key, w0);
-    or_six_uniforms(w0, u0);
+    or_block_draws(w0, u0, p0);
     REAL lambda = FN(or_wavelength_)(sp, knots, (REAL)u0[0]);
     REAL inv_dl = (REAL)((double)(med->n_points - 1) /
                          ((double)med->lambda_max - (double)med->lambda_min));
     REAL n_phase = FN(or_table_)(med->n_phase, med->n_points, (lambda - (REAL)med->lambda_min) * inv_dl);
     REAL dir[3];
-    FN(or_init_direction_)(em, axis, n_phase, (REAL)u0[1], (REAL)u0[2], dir);
+    FN(or_init_direction_)(em, axis, n_phase, (REAL)p0[0], (REAL)u0[1], dir);
     out[0] = lambda;
     out[1] = dir[0];
     out[2] = dir[1];
@@ -217,12 +217,15 @@ static void FN(or_photon_)(const pp_medium_tables* med, const pp_spectrum* sp, c
                            FN(or_acc_) * acc) {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t ctr[4] = {0u, (uint32_t)pid, (uint32_t)(pid >> 32), stream};
-    /* Draws: block 0 = {λ, φ, θ, step 0 (3)}; block b >= 1 = {step 2b-1 (3),
-     * step 2b (3)}; six 21-bit uniforms per block (or_six_uniforms). */
+    /* Draws (or_block_draws) per Philox block (w0, w1, w2, w3):
+     *   block 0:      λ <- u0, emission θ <- u1, emission φ <- p0;
+     *                 step 0 <- {u2, u3, p1};
+     *   block b >= 1: step 2b-1 <- {u0, u1, p0}, step 2b <- {u2, u3, p1};
+     * {step length, scattering angle, azimuth} per step. */
     uint32_t w0[4];
-    float ub[6];
+    float ub[4], pb[2];
     or_philox4x32_10(ctr, key, w0);
-    or_six_uniforms(w0, ub);
+    or_block_draws(w0, ub, pb);
 
     REAL u_lambda = (REAL)ub[0];
     REAL lambda = FN(or_wavelength_)(sp, knots, u_lambda);
@@ -235,7 +238,7 @@ static void FN(or_photon_)(const pp_medium_tables* med, const pp_spectrum* sp, c
     REAL abs_len = FN(or_table_)(med->abs_len, med->n_points, xs);
 
     REAL dir[3], init_dir[3];
-    FN(or_init_direction_)(em, axis, n_phase, (REAL)ub[1], (REAL)ub[2], dir);
+    FN(or_init_direction_)(em, axis, n_phase, (REAL)pb[0], (REAL)ub[1], dir);
     init_dir[0] = dir[0];
     init_dir[1] = dir[1];
     init_dir[2] = dir[2];
@@ -252,14 +255,14 @@ static void FN(or_photon_)(const pp_medium_tables* med, const pp_spectrum* sp, c
 
     for (uint32_t s = 0; s < max_steps; ++s) {
         const uint32_t blk = (s + 1u) / 2u;
-        const uint32_t off = s == 0 ? 3u : ((s + 1u) % 2u) * 3u;
+        const uint32_t slot = (s % 2u == 0u) ? 1u : 0u; /* even steps: {u2, u3, p1} */
         if (blk != have_block) {
             ctr[0] = blk;
             or_philox4x32_10(ctr, key, words);
-            or_six_uniforms(words, ub);
+            or_block_draws(words, ub, pb);
             have_block = blk;
         }
-        REAL u[3] = {(REAL)ub[off], (REAL)ub[off + 1], (REAL)ub[off + 2]};
+        REAL u[3] = {(REAL)ub[2 * slot], (REAL)ub[2 * slot + 1], (REAL)pb[slot]};
         ++steps;
         REAL step = -R_LOG(u[0]) * sca; /* src/photon_prop_cuda.jl:322-323 */
         REAL d;

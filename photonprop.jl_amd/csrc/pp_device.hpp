@@ -55,19 +55,24 @@ __device__ __forceinline__ float u01(uint32_t w) {
     return __builtin_fmaf((float)(w >> 8), 0x1p-24f, 0x1p-24f);
 }
 
-// The propagation takes six 21-bit uniforms from each Philox block: the top 21
-// bits of words x, y, z, w, then (x & 0x7FF) | (y & 0x3FF) << 11 and
-// (z & 0x7FF) | (w & 0x3FF) << 11.  u21(v) = (v + 1) * 2^-21, exact in FP32.
-__device__ __forceinline__ float u21(uint32_t v) {
-    return __builtin_fmaf((float)v, 0x1p-21f, 0x1p-21f);
+// The propagation's draws from one Philox block (words w0..w3):
+//   a_k = 1 + (w_k >> 9)·2^-23 on [1, 2), exact in FP32: the word's top 23 bits
+//         as the mantissa under the exponent of 1.0 (one v_alignbit_b32); the
+//         matching uniform on (0, 1] is u_k = 2 - a_k, also exact;
+//   p0  = ((w0 & 0xFF) << 8 | (w1 & 0xFF))·2^-16 and p1 likewise from w2, w3:
+//         16-bit azimuths in revolutions on [0, 1) (one v_perm_b32), the unit
+//         the hardware sin/cos take.
+__device__ __forceinline__ float draw_a(uint32_t w) {
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, w, 9u));
 }
-__device__ __forceinline__ uint32_t v21_hi(uint32_t w) { return w >> 11; }
-__device__ __forceinline__ uint32_t v21_lo(uint32_t a, uint32_t b) {
-    return (a & 0x7FFu) | ((b & 0x3FFu) << 11);
+__device__ __forceinline__ float draw_p(uint32_t hi, uint32_t lo) {
+    return (float)__builtin_amdgcn_perm(hi, lo, 0x0C0C0400u) * 0x1p-16f;
 }
 
 // Henyey-Greenstein cosθ by inverse CDF (src/photon_prop_cuda.jl:31-38):
 // cosθ = (1+g² - ((1-g²)/(1+g(2η-1)))²)/(2g), clamped; g == 0 is isotropic.
+// The uniform comes as a = 2 - η on [1, 2), so 2η-1 = 3-2a, exact in FP32;
+// the denominator 1 + g(2η-1) is then rounded once, as in the reference form.
 struct HG {
     float g, num, opg2, inv2g;
     int iso;
@@ -81,6 +86,14 @@ __device__ __forceinline__ float hg_costheta(const HG& h, float eta) {
     float c = h.inv2g * (h.opg2 - q * q);
     c = fminf(fmaxf(c, -1.0f), 1.0f);
     return c;
+}
+
+__device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
+    const float x = __builtin_fmaf(-2.0f, a, 3.0f);
+    if (h.iso) return x;
+    const float q = h.num * __builtin_amdgcn_rcpf(__builtin_fmaf(h.g, x, 1.0f));
+    const float c = h.inv2g * __builtin_fmaf(-q, q, h.opg2);
+    return __builtin_amdgcn_fmed3f(c, -1.0f, 1.0f);
 }
 
 // rotate_to_axis (src/photon_prop_cuda.jl:171-219): apply to (nx,ny,nz) the

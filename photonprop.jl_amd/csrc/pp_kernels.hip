@@ -151,7 +151,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     const float r2 = R.r2;
     const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
 
-    uint32_t wave_steps = 0, acc_hits = 0;  // wave_steps is wave-uniform
+    uint32_t acc_steps = 0, acc_hits = 0;
     double acc_w = 0.0, acc_wt = 0.0, acc_wt2 = 0.0;
 
     for (uint32_t k = 0; k < rounds; ++k) {
@@ -161,9 +161,14 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         const uint32_t id_lo = (uint32_t)pid, id_hi = (uint32_t)(pid >> 32);
 
         // ---- emission (src/photon_prop_cuda.jl:308-316) ----
-        // Draws: block 0 = {λ, φ, θ, step 0}; block b >= 1 = {step 2b-1, step 2b}.
+        // Draws (ppd::draw_a, ppd::draw_p) per Philox block (x, y, z, w):
+        //   block 0:      λ <- a(x), emission θ <- a(y), emission φ <- p(x, y);
+        //                 step 0 <- {a(z), a(w), p(z, w)}
+        //   block b >= 1: step 2b-1 <- {a(x), a(y), p(x, y)},
+        //                 step 2b   <- {a(z), a(w), p(z, w)}
+        // with {step length, scattering angle, azimuth} per step.
         const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-        const float u_l = ppd::u21(ppd::v21_hi(w0.x));
+        const float u_l = 2.0f - ppd::draw_a(w0.x);
         float lambda;
         if (T.spec.kind == PP_SPEC_MONO) {
             lambda = T.spec.mono;
@@ -179,10 +184,10 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
 
         float dx, dy, dz;
         {
-            const float u_phi = ppd::u21(ppd::v21_hi(w0.y));
+            const float u_phi = ppd::draw_p(w0.x, w0.y);
             const float sp = __builtin_amdgcn_sinf(u_phi);
             const float cp = __builtin_amdgcn_cosf(u_phi);
-            const float u_th = ppd::u21(ppd::v21_hi(w0.z));
+            const float u_th = 2.0f - ppd::draw_a(w0.y);
             if (R.profile == PP_EMIT_REF_ISOTROPIC) {
                 const float theta = __builtin_fmaf(u_th, 2.0f, -1.0f);
                 const float st = __builtin_amdgcn_sinf(theta * kInv2Pi);
@@ -206,12 +211,13 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         float dist = 0.0f;
         bool alive = active;
         bool hit = false;
+        // steps of this photon: max_steps unless it hits at step k (k + 1)
+        uint32_t nst = active ? max_steps : 0u;
 
         // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
-#define PP_STEP(WA, WB, WC)                                                              \
+#define PP_STEP(K, AS, AC, PH)                                                           \
     if (alive) {                                                                        \
-        const float u1 = ppd::u21(WA);                                                  \
-        float s = -__builtin_amdgcn_logf(u1) * sca_ln2;                                 \
+        float s = -__builtin_amdgcn_logf(2.0f - (AS)) * sca_ln2;                        \
         const float a = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));        \
         const float pp = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));       \
         const float b = __builtin_fmaf(a, a, r2 - pp);                                  \
@@ -228,36 +234,31 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         qz = __builtin_fmaf(s, dz, qz);                                                 \
         dist += s;                                                                      \
         if (h) {                                                                        \
+            nst = (K) + 1u;                                                             \
             hit = true;                                                                 \
             alive = false;                                                              \
         } else {                                                                        \
-            const float ct = ppd::hg_costheta(T.hg, ppd::u21(WB));                      \
+            const float ct = ppd::hg_costheta_a(T.hg, (AC));                            \
             const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));     \
-            const float uph = ppd::u21(WC);                                             \
-            const float sph = __builtin_amdgcn_sinf(uph);                               \
-            const float cph = __builtin_amdgcn_cosf(uph);                               \
+            const float sph = __builtin_amdgcn_sinf(PH);                                \
+            const float cph = __builtin_amdgcn_cosf(PH);                                \
             ppd::rotate_to_axis(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);        \
         }                                                                               \
     }
 
-        // Steps are counted per wave from the alive ballot (scalar unit).
-        uint64_t am = __ballot(alive);
-        wave_steps += (uint32_t)__popcll(am);
-        PP_STEP(ppd::v21_hi(w0.w), ppd::v21_lo(w0.x, w0.y), ppd::v21_lo(w0.z, w0.w))
+        // No per-step ballot: a wave whose lanes have all hit skips the
+        // (divergent) step bodies and Philox blocks on an empty exec mask.
+        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w))
         for (uint32_t s0 = 1; s0 < max_steps; s0 += 2) {
-            am = __ballot(alive);
-            if (am == 0) break;
-            wave_steps += (uint32_t)__popcll(am);
-            const ppd::U4 Q =
-                ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(ppd::v21_hi(Q.x), ppd::v21_hi(Q.y), ppd::v21_hi(Q.z))
+            ppd::U4 Q;
+            if (alive)
+                Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y))
             if (s0 + 1 >= max_steps) break;
-            am = __ballot(alive);
-            if (am == 0) break;
-            wave_steps += (uint32_t)__popcll(am);
-            PP_STEP(ppd::v21_hi(Q.w), ppd::v21_lo(Q.x, Q.y), ppd::v21_lo(Q.z, Q.w))
+            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w))
         }
 #undef PP_STEP
+        acc_steps += nst;
 
         // ---- hit append: one atomic per wave with any hit ----
         const uint64_t mask = __ballot(hit);
@@ -311,7 +312,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     }
 
     // ---- tile partial: fixed-order reduction ----
-    const uint32_t ws = wave_steps;
+    const uint32_t ws = wave_sum_u32(acc_steps);
     const uint32_t wh = wave_sum_u32(acc_hits);
     const double w1 = wave_sum_f64(acc_w);
     const double w2 = wave_sum_f64(acc_wt);
@@ -362,7 +363,7 @@ __device__ __forceinline__ TileCtx stage_tables(float4* s_table, const float4* t
     return T;
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) propagate_kernel(KParams P) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) propagate_kernel(KParams P) {
     extern __shared__ float4 s_table[];  // n_points float4, then knots
     __shared__ uint32_t s_tile;
     __shared__ double s_red[kWaves][3];
@@ -604,7 +605,7 @@ __device__ __forceinline__ uint32_t hg_word(uint64_t i, uint32_t k0, uint32_t k1
 __global__ void selftest_hg_kernel(ppd::HG hg, uint32_t k0, uint32_t k1, float* out, uint64_t n) {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = ppd::hg_costheta(hg, ppd::u01(hg_word(i, k0, k1)));
+    out[i] = ppd::hg_costheta_a(hg, ppd::draw_a(hg_word(i, k0, k1)));
 }
 
 __global__ void __launch_bounds__(256) selftest_hg_moments_kernel(ppd::HG hg, uint32_t k0,
@@ -614,7 +615,7 @@ __global__ void __launch_bounds__(256) selftest_hg_moments_kernel(ppd::HG hg, ui
     double a = 0.0, b = 0.0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const double c = (double)ppd::hg_costheta(hg, ppd::u01(hg_word(i, k0, k1)));
+        const double c = (double)ppd::hg_costheta_a(hg, ppd::draw_a(hg_word(i, k0, k1)));
         a += c;
         b += c * c;
     }

@@ -106,7 +106,9 @@ def test_device_hg_per_draw_matches_oracle_and_cdf(ctx):
     out = np.zeros(n, dtype=np.float32)
     pp._abi.check(ctx.lib.pp_selftest_hg(ctx.handle, g, 3, pp._abi.ptr(out), n))
     words = np.concatenate([oracle.philox4x32_10([i, 0, 0, 0], [3, 0]) for i in range(n // 4)])
-    ref = np.array([oracle.hg_costheta(g, oracle.uniform(int(w))) for w in words])
+    # the propagation's draw: eta = 1 - (w >> 9)·2^-23 (exact in FP32)
+    ref = np.array([oracle.hg_costheta(g, 1.0 - (int(w) >> 9) * 2.0 ** -23) for w in words])
+    # same algebra, FP32; the device's reciprocal is the 1-ulp v_rcp_f32
     np.testing.assert_allclose(out, ref, atol=3e-6)
     cdf = lambda mu: (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * mu) - 1 / (1 + g))
     assert stats.kstest(out.astype(np.float64), cdf).pvalue > 0.01

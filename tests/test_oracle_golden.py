@@ -47,24 +47,36 @@ def test_uniform_range_and_ends():
 
 # --------------------------------------------------------------- uniform ----
 
-def test_six_uniforms_range_uniformity_and_independence():
-    """The propagation draws six 21-bit uniforms per Philox block: each must be
-    on (0, 1] with the 2^-21 lattice, uniform (KS), and the two assembled from
-    low bits must not correlate with the four from high bits."""
+def test_block_draws_range_uniformity_and_independence():
+    """Per Philox block the propagation draws four 23-bit uniforms on (0, 1]
+    (the words' top 23 bits) and two 16-bit azimuths on [0, 1) (the low bytes
+    of word pairs): each on its lattice, uniform (KS), and the six mutually
+    uncorrelated (bits 8 of the words are the only ones left unused)."""
     blocks = [oracle.philox4x32_10([i, 7, 0, 3], [11, 0]) for i in range(20000)]
-    u = np.array([oracle.six_uniforms(b) for b in blocks], dtype=np.float64)
-    assert u.min() > 0.0 and u.max() <= 1.0
-    assert np.all(np.abs(u * 2 ** 21 - np.round(u * 2 ** 21)) == 0)
+    d = [oracle.block_draws(b) for b in blocks]
+    u = np.array([x[0] for x in d], dtype=np.float64)
+    p = np.array([x[1] for x in d], dtype=np.float64)
+    assert u.min() > 0.0 and u.max() <= 1.0 and p.min() >= 0.0 and p.max() < 1.0
+    assert np.all(u * 2 ** 23 == np.round(u * 2 ** 23))
+    assert np.all(p * 2 ** 16 == np.round(p * 2 ** 16))
+    allv = np.concatenate([u, p], axis=1)
     for j in range(6):
-        assert stats.kstest(u[:, j], "uniform").pvalue > 0.001, j
-    c = np.corrcoef(u.T)
+        assert stats.kstest(allv[:, j], "uniform").pvalue > 0.001, j
+    c = np.corrcoef(allv.T)
     assert np.abs(c - np.eye(6)).max() < 0.04
-    # all-zero and all-one words hit the lattice ends
-    np.testing.assert_array_equal(oracle.six_uniforms([0, 0, 0, 0]), [2.0 ** -21] * 6)
-    np.testing.assert_array_equal(oracle.six_uniforms([0xFFFFFFFF] * 4), [1.0] * 6)
-    # v4 takes the low 11 bits of word 0 and bits 0..9 of word 1
-    np.testing.assert_array_equal(oracle.six_uniforms([0x7FF, 0x3FF, 0, 0])[4], 1.0)
-    assert oracle.six_uniforms([0, 0x400, 0, 0])[4] == 2.0 ** -21  # bit 10 of word 1 unused
+    # lattice ends: word 0 -> u = 1 (the device's a = 1), all ones -> u = 2^-23
+    u0, p0 = oracle.block_draws([0, 0, 0, 0])
+    np.testing.assert_array_equal(u0, [1.0] * 4)
+    np.testing.assert_array_equal(p0, [0.0, 0.0])
+    u1, p1 = oracle.block_draws([0xFFFFFFFF] * 4)
+    np.testing.assert_array_equal(u1, [2.0 ** -23] * 4)
+    np.testing.assert_array_equal(p1, [1.0 - 2.0 ** -16] * 2)
+    # u_k uses bits 9..31 only; p0 = byte 0 of word 0 (high) and of word 1 (low)
+    assert oracle.block_draws([0x1FF, 0, 0, 0])[0][0] == 1.0
+    assert oracle.block_draws([0x200, 0, 0, 0])[0][0] == 1.0 - 2.0 ** -23
+    assert oracle.block_draws([0x01, 0x00, 0, 0])[1][0] == 2.0 ** -8
+    assert oracle.block_draws([0x00, 0x01, 0, 0])[1][0] == 2.0 ** -16
+    assert oracle.block_draws([0x100, 0x100, 0, 0])[1][0] == 0.0  # bit 8 unused
 
 def test_uniform_range_reference_kat():
     # src/tests/test_photon_prob.jl:13-17: uniform(2, 5) stays in [2, 5].
@@ -245,13 +257,16 @@ def test_wavelength_inverse_square_closed_form():
     t, sp, em, keep = _structs(m, pp.InverseSquareSpectrum(300, 600), "CherenkovCone")
     out = oracle.emit(t, sp, em, 0, 0, np.arange(4000), precision=64)
     lam = out[:, 0]
-    u = np.array([oracle.six_uniforms(oracle.philox4x32_10([0, i, 0, 0], [0, 0]))[0]
+    u = np.array([oracle.block_draws(oracle.philox4x32_10([0, i, 0, 0], [0, 0]))[0][0]
                   for i in range(4000)], dtype=np.float64)
     np.testing.assert_allclose(lam, 600.0 / (2.0 - u), rtol=1e-12)
     # KS against the closed-form CDF, and the mean 600·ln 2
     cdf = pp.InverseSquareSpectrum(300, 600).cdf
     assert stats.kstest(lam, cdf).pvalue > 0.01
-    assert abs(lam.mean() - 600 * math.log(2)) < 2.0
+    # mean 600·ln 2; Var λ = 600·300 - (600 ln 2)² (density 600/λ² on [300, 600]),
+    # so the mean of 4000 draws has sigma 83.8/sqrt(4000) = 1.33 nm: 4 sigma
+    sigma = math.sqrt(600.0 * 300.0 - (600 * math.log(2)) ** 2) / math.sqrt(lam.size)
+    assert abs(lam.mean() - 600 * math.log(2)) < 4 * sigma
 
 
 def test_cherenkov_cone_angle_and_azimuth():
