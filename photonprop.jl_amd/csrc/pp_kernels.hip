@@ -1187,6 +1187,7 @@ constexpr uint64_t kSrvExit = ~0ull;
 constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
 constexpr size_t kSrvGateOffset = 4096;  // ticket and gate words on separate pages
 constexpr int kSrvVgprReserve = 64;      // VGPRs per SIMD lane left for other kernels
+constexpr int kSrvMaxWavesPerSimd = 4;   // of 8 wave slots per SIMD: 4 left for other kernels
 
 struct SrvEntry {
     uint64_t t0, t1;
@@ -1691,16 +1692,18 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
              (s->spec.kind == PP_SPEC_KNOTS ? (size_t)s->spec.n_knots * sizeof(float) : 0);
     // The persistent kernel must leave room on every CU for other kernels: the
     // runtime's copy kernels behind pp_server_result's device-to-host copy
-    // among them.  If it filled the register file, a result copy would wait
-    // for the kernel to exit while the kernel waits (busy) for the request
-    // that the copy's pool release would admit.  Each block puts one wave on
-    // each SIMD; keep kSrvVgprReserve of the 512 VGPRs per lane free.
+    // among them.  If it filled a CU, a result copy would wait for the kernel
+    // to exit while the kernel waits (busy) for the request that the copy's
+    // pool release would admit.  Each block puts one wave on each SIMD; keep
+    // kSrvVgprReserve of the 512 VGPRs per lane and half the 8 wave slots free
+    // (a 64-VGPR build at 7 blocks per CU left 64 VGPRs but one wave slot and
+    // deadlocked this way).
     int per_cu = 0;
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, s->lds));
     hipFuncAttributes fa;
     PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
     const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
-    per_cu = std::min(per_cu, (512 - kSrvVgprReserve) / vgpr);
+    per_cu = std::min({per_cu, (512 - kSrvVgprReserve) / vgpr, kSrvMaxWavesPerSimd});
     s->grid = std::max(1, per_cu) * c->n_cu;
     s->free_slots.reserve(S);
     for (uint32_t i = 0; i < S; ++i) s->free_slots.push_back(S - 1 - i);
