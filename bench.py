@@ -37,25 +37,11 @@ PROFILED_TRAFFIC_BYTES = 2.188e10
 
 
 def workload_requests(pp, photons_per_geometry, seed=0):
-    dists = np.geomspace(1.0, 100.0, 16)
-    angles = (0.0, 45.0, 90.0, 135.0)
-    reqs = pp.make_requests(len(dists) * len(angles))
-    i = 0
-    for d in dists:
-        for a in angles:
-            ar = math.radians(a)
-            reqs["pos"][i] = (0.0, 0.0, 0.0)
-            reqs["axis"][i] = (0.0, 0.0, 1.0)
-            reqs["beta"][i] = 1.0
-            reqs["n_photons"][i] = photons_per_geometry
-            reqs["profile"][i] = pp._abi.PP_EMIT_CHERENKOV_CONE
-            reqs["center"][i] = (d * math.sin(ar), 0.0, d * math.cos(ar))
-            reqs["radius"][i] = 0.21
-            reqs["seed"][i] = seed
-            reqs["stream"][i] = i
-            reqs["max_steps"][i] = 10
-            i += 1
-    return reqs
+    """The 64 geometries: 16 distances geomspace(1, 100, 16) m x targets at
+    0/45/90/135 degrees from the +z axis, r = 0.21 m, max_steps 10."""
+    from photonprop import workloads
+
+    return workloads.scale_geometries(photons=photons_per_geometry, radius=0.21, seed=seed)
 
 
 def cpu_baseline(pp, photons_per_geometry_sample):

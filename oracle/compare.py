@@ -39,7 +39,7 @@ def table_struct(tables):
 
 
 def oracle_for(ctx, source, target, seed=0, stream=0, max_steps=10, id_begin=0, id_end=None,
-               precision=32, n_threads=None):
+               precision=32, n_threads=None, hits_cap=None):
     """Run the oracle with the same inputs a Context would send to the device.
     `ctx` is a photonprop.Context or a photonprop.MediumTables."""
     from photonprop import _abi
@@ -51,7 +51,7 @@ def oracle_for(ctx, source, target, seed=0, stream=0, max_steps=10, id_begin=0, 
     tg = sphere_struct(target)
     hits, summ = oracle_propagate(tables, spec, em, tg, seed=seed, stream=stream,
                                   max_steps=max_steps, id_begin=id_begin, id_end=id_end,
-                                  precision=precision, n_threads=n_threads,
+                                  precision=precision, n_threads=n_threads, hits_cap=hits_cap,
                                   hit_dtype=_abi.HIT_DTYPE, summary_struct=_abi.pp_summary)
     del keep, tkeep
     return hits, {k: getattr(summ, k) for k, _ in _abi.pp_summary._fields_}

@@ -338,3 +338,50 @@ def test_invalid_inputs_raise(ctx):
     # zero photons: an empty, exact result
     res = ctx.propagate(cone_source(0), tgt)
     assert res.summary["n_sim"] == 0 and res.summary["n_hits"] == 0 and res.hits.shape[0] == 0
+
+
+def test_scale_geometries_statistics_vs_fp64_oracle(ctx):
+    """The 64 bench geometries (16 distances geomspace(1, 100, 16) m x targets
+    at 0/45/90/135 degrees from the emitter axis, r = 0.21 m): the batched kernel
+    at 1e8 photons per geometry (seed 1, up to 20000 hit records kept per
+    geometry) against the FP64 oracle at 1e7 (seed 2).  For every geometry with
+    >= 200 oracle hits: the yield Σw/N within 4 sigma (the oracle's error from its
+    hit sample, the GPU's from its hit count and the oracle's per-hit E[w²]) and
+    a two-sample KS test of the arrival times with p > 1e-4; over all of them the
+    yields' Σz² follows χ²(k) and Fisher's combination of the KS p-values
+    χ²(2k), each with p > 0.001.  (The arrival times are heavy-tailed, so their
+    mean is compared through the whole distribution rather than a z-score.)"""
+    from photonprop import workloads
+
+    n_gpu, n_cpu, keep = 100_000_000, 10_000_000, 20_000
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = workloads.scale_geometries(photons=n_gpu, seed=1)
+    gh, off, gs = ctx.propagate_batch(reqs, spec, per_request_cap=keep, hits_cap=len(reqs) * keep * 40,
+                                      allow_overflow=True)
+    assert np.all(gs["n_sim"] == n_gpu)
+    zs, ps = [], []
+    for i, r in enumerate(reqs):
+        src = cone_source(n_cpu)
+        tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
+        ch, cs = oracle_for(ctx, src, tgt, seed=2, stream=i, precision=64, hits_cap=2_000_000)
+        if cs["n_hits"] < 200:
+            continue
+        assert cs["n_stored"] == cs["n_hits"]
+        w = ch["weight"].astype(np.float64)
+        ng = int(gs["n_hits"][i])
+        yc, yg = cs["sum_w"] / n_cpu, gs["sum_w"][i] / n_gpu
+        vc = ((w ** 2).sum() / n_cpu - yc ** 2) / n_cpu
+        vg = ((w ** 2).mean() * ng / n_gpu - yg ** 2) / n_gpu
+        zy = (yg - yc) / math.sqrt(vc + vg)
+        tg = gh["time"][int(off[i]):int(off[i + 1])].astype(np.float64)
+        p = stats.ks_2samp(tg, ch["time"].astype(np.float64)).pvalue
+        say("geom %2d d=%6.2f hits gpu %8d (kept %5d) cpu %7d  yield z=%+.2f  KS(time) p=%.3f"
+            % (i, np.linalg.norm(r["center"]), ng, tg.size, cs["n_hits"], zy, p))
+        assert abs(zy) < 4 and p > 1e-4, (i, zy, p)
+        zs.append(zy)
+        ps.append(p)
+    k = len(zs)
+    p_y = stats.chi2.sf(float(np.sum(np.square(zs))), k)
+    p_t = stats.chi2.sf(float(-2.0 * np.sum(np.log(ps))), 2 * k)
+    say("geometries compared %d: yield chi2 p %.3f, Fisher KS p %.3f" % (k, p_y, p_t))
+    assert k >= 20 and p_y > 0.001 and p_t > 0.001

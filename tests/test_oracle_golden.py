@@ -359,10 +359,12 @@ def test_straight_line_survival_exp_minus_d_over_abs():
     assert abs(summ.sum_w / src.photons - math.exp(-(d - r) / la)) < 1e-3
 
 
-def test_unscattered_fraction_exp_minus_d_over_sca():
+@pytest.mark.parametrize("d", [0.8, 2.3, 6.0, 15.3])
+def test_unscattered_fraction_exp_minus_d_over_sca(d):
     # With one step allowed, a photon on the axis reaches the sphere iff its first
-    # exponential step exceeds d - r: P = exp(-(d - r)/λ_sca).
-    n, d, r, ls = 1.34, 6.0, 0.3, 8.0
+    # exponential step exceeds d - r: P = exp(-(d - r)/λ_sca).  The four distances
+    # pin the step-length CDF at P = 0.94, 0.76, 0.49 and 0.15.
+    n, r, ls = 1.34, 0.3, 8.0
     med = _Med(n=n, abs_len=1e30, sca_len=ls)
     src = pp.PhotonSource((0, 0, 0), (0, 0, 1), 0.0, 100_000, pp.Monochromatic(450.0),
                           pp.AngularEmissionProfile("CherenkovCone"), beta=1.0 / n)
