@@ -308,6 +308,9 @@ def main():
                     help="MC training rows for the surrogate measurement (0 skips it)")
     ap.add_argument("--surrogate-photons", type=float, default=1e5, help="photons per training row")
     ap.add_argument("--surrogate-epochs", type=int, default=40)
+    ap.add_argument("--mlp-probe", type=float, default=0,
+                    help="only time the surrogate's fused forward on this many random rows "
+                         "(random weights; for profiling) and exit")
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter "
                          "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -322,6 +325,24 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": 4 * n, "dispatches": 3}))
+        return
+    if args.mlp_probe:
+        import torch
+
+        import photonprop as pp
+        from photonprop import surrogate as sg
+
+        n = int(args.mlp_probe)
+        ctx = pp.Context(pp.SeawaterConfigMedium(), device=0)
+        mlp = sg.MLP(ctx, seed=0)
+        xq = torch.rand((n, 8), device="cuda:0") * 2 - 1
+        yq = torch.empty((n, 10), device="cuda:0")
+        times = [mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n) for _ in range(5)]
+        fl = n * 2.0 * (8 * 256 + 3 * 256 * 256 + 256 * 10)
+        print(json.dumps({"mlp_probe_rows": n, "forward_ms": times,
+                          "queries_per_s": n / (1e-3 * min(times)),
+                          "tflops": fl / (1e-3 * min(times)) / 1e12}))
+        mlp.close()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU; --gpus %d is not supported (the path shards by photon-id "
