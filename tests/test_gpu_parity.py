@@ -156,6 +156,30 @@ def test_per_photon_parity(ctx, case):
         assert np.all(res.hits["weight"] > 0) and np.all(res.hits["weight"] <= 1)
 
 
+def test_photon_ids_across_two_to_the_32(ctx):
+    """A request of 2^32 + 5e6 photons: the Philox counter's high id word and
+    the 64-bit tile bases come into play.  The hits of photon ids in
+    [2^32 - 5e6, 2^32 + 5e6) agree photon by photon with the oracle run on that
+    id range alone, and the counts are exact."""
+    n = (1 << 32) + 5_000_000
+    a = math.radians(45.0)
+    tgt = pp.DetectionSphere((21.54 * math.sin(a), 0.0, 21.54 * math.cos(a)), 0.21)
+    src = cone_source(n)
+    res = ctx.propagate(src, tgt, seed=3, stream=1, hits_cap=2_000_000)
+    s = res.summary
+    say("2^32+5e6 photons:", s)
+    assert s["n_sim"] == n and s["n_stored"] == s["n_hits"] > 100_000
+    assert 9.0 * n < s["n_steps"] <= 10 * n
+    lo, hi = (1 << 32) - 5_000_000, (1 << 32) + 5_000_000
+    ids = res.hits["photon_id"]
+    mine = res.hits[(ids >= lo) & (ids < hi)]
+    ref, rsum = oracle_for(ctx, src, tgt, seed=3, stream=1, id_begin=lo, id_end=hi)
+    cmp = compare_hits(mine, ref)
+    say("window [2^32 - 5e6, 2^32 + 5e6):", cmp)
+    assert rsum["n_hits"] > 300 and np.any(ref["photon_id"] >= (1 << 32))
+    assert_parity(cmp, rsum["n_hits"])
+
+
 def test_knot_spectrum_and_reference_medium_parity():
     # The reference's own setup: cascadia water, CherenkovSpectrum((300, 800), 20),
     # :IsotropicEmission, g = 0.99, sphere on the z axis (src/photon_prop_cuda.jl:588-609).
