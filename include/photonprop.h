@@ -231,6 +231,11 @@ int pp_last_kernel_ms(pp_ctx* ctx, float* ms);
 
 #define PP_TILE_PHOTONS 8192
 
+/* Largest hit-record capacity any call accepts (hits_cap, hit_pool_records):
+ * 2^36 records of 64 B, beyond any device; larger values are rejected with
+ * PP_ERR_INVALID_ARG rather than wrapped in a size computation. */
+#define PP_MAX_HIT_RECORDS (1ull << 36)
+
 typedef struct pp_server_config {
     uint32_t max_slots;             /* requests on the device at once, 1..4096 */
     uint32_t max_tiles_per_request; /* photons per request <= this * PP_TILE_PHOTONS */

@@ -897,9 +897,17 @@ int validate_ctx(pp_ctx* c) {
     return PP_OK;
 }
 
+int check_records(uint64_t n, const char* what) {
+    if (n > PP_MAX_HIT_RECORDS)
+        return fail(PP_ERR_INVALID_ARG, "%s %llu exceeds PP_MAX_HIT_RECORDS (%llu)", what,
+                    (unsigned long long)n, (unsigned long long)PP_MAX_HIT_RECORDS);
+    return PP_OK;
+}
+
 int run_batch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spectrum* spec,
               pp_hit* d_out, uint64_t hits_cap, uint64_t per_cap, uint64_t* d_offsets) {
     int rc;
+    if ((rc = check_records(hits_cap, "hits_cap"))) return rc;
     if ((rc = ensure(c->staging, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
     if ((rc = enqueue_propagation(c, reqs, n_req, spec, (pp_hit*)c->staging.p, hits_cap))) return rc;
     return enqueue_compaction(c, n_req, (const pp_hit*)c->staging.p, hits_cap, per_cap, d_out,
@@ -908,7 +916,9 @@ int run_batch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spectr
 
 // Records the compacted output of a batch can hold.
 uint64_t out_records(uint64_t hits_cap, uint64_t per_cap, uint32_t n_req) {
-    return per_cap ? std::min<uint64_t>(hits_cap, per_cap * n_req) : hits_cap;
+    if (!per_cap) return hits_cap;
+    const unsigned __int128 all = (unsigned __int128)per_cap * n_req;  // no wrap-around
+    return all < hits_cap ? (uint64_t)all : hits_cap;
 }
 
 }  // namespace
@@ -991,6 +1001,7 @@ int pp_propagate(pp_ctx* c, const pp_emitter* emitter, const pp_spectrum* spectr
     if (rc) return rc;
     if (!emitter || !target) return fail(PP_ERR_INVALID_ARG, "emitter/target is NULL");
     if (hits_cap && !hits_out) return fail(PP_ERR_INVALID_ARG, "hits_out is NULL with hits_cap > 0");
+    if ((rc = check_records(hits_cap, "hits_cap"))) return rc;
     pp_request q;
     memset(&q, 0, sizeof q);
     q.emitter = *emitter;
@@ -1027,6 +1038,7 @@ int pp_propagate_batch(pp_ctx* c, const pp_request* requests, uint32_t n_req,
     if (hits_cap && !hits_out) return fail(PP_ERR_INVALID_ARG, "hits_out is NULL with hits_cap > 0");
     if (!offsets_out || !summaries_out)
         return fail(PP_ERR_INVALID_ARG, "offsets_out/summaries_out is NULL");
+    if ((rc = check_records(hits_cap, "hits_cap"))) return rc;
     const uint64_t out_cap = out_records(hits_cap, per_request_cap, n_req);
     if ((rc = ensure(c->out_hits, std::max<uint64_t>(out_cap, 1) * sizeof(pp_hit)))) return rc;
     if ((rc = ensure(c->offsets, ((uint64_t)n_req + 1) * sizeof(uint64_t)))) return rc;
@@ -1726,6 +1738,7 @@ int pp_server_create(pp_ctx* c, const pp_spectrum* spectrum, const pp_server_con
     *out = nullptr;
     if (cfg->max_slots < 1 || cfg->max_slots > 4096)
         return fail(PP_ERR_INVALID_ARG, "max_slots %u not in [1, 4096]", cfg->max_slots);
+    if ((rc = check_records(cfg->hit_pool_records, "hit_pool_records"))) return rc;
     if (cfg->max_tiles_per_request < 1 || cfg->max_tiles_per_request > (1u << 20))
         return fail(PP_ERR_INVALID_ARG, "max_tiles_per_request %u not in [1, 2^20]",
                     cfg->max_tiles_per_request);

@@ -117,6 +117,7 @@ class pp_server_stats(ctypes.Structure):
 
 
 PP_TILE_PHOTONS = 8192
+PP_MAX_HIT_RECORDS = 1 << 36
 
 # pp_hit as a numpy record (64 bytes).
 HIT_DTYPE = np.dtype([

@@ -359,6 +359,19 @@ def test_invalid_inputs_raise(ctx):
         ctx.propagate(cone_source(10, beta=0.5), tgt)
     with pytest.raises(pp.PhotonPropError, match="max_steps"):
         ctx.propagate(cone_source(10), tgt, max_steps=0)
+    # a capacity whose byte size would wrap around 2^64 is refused before any
+    # allocation or launch (raw call: the wrapper would allocate the buffer)
+    import ctypes
+    from photonprop.propagation import emitter_struct, spectrum_struct, sphere_struct
+    spec, keep = spectrum_struct(pp.InverseSquareSpectrum(300.0, 600.0))
+    small = np.zeros(4, dtype=pp._abi.HIT_DTYPE)
+    summ = pp._abi.pp_summary()
+    rc = ctx.lib.pp_propagate(ctx.handle, ctypes.byref(emitter_struct(cone_source(10))),
+                              ctypes.byref(spec), ctypes.byref(sphere_struct(tgt)), 0, 0, 10,
+                              pp._abi.ptr(small), 1 << 58, ctypes.byref(summ))
+    assert rc == pp._abi.PP_ERR_INVALID_ARG and b"hits_cap" in ctx.lib.pp_last_error()
+    with pytest.raises(pp.PhotonPropError, match="hit_pool_records"):
+        pp.Server(ctx, pp.InverseSquareSpectrum(300.0, 600.0), hit_pool_records=1 << 40)
     # zero photons: an empty, exact result
     res = ctx.propagate(cone_source(0), tgt)
     assert res.summary["n_sim"] == 0 and res.summary["n_hits"] == 0 and res.hits.shape[0] == 0
