@@ -31,15 +31,21 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 
 // Philox4x32-10.  The key (k0, k1) is wave-uniform, so its schedule lives in
 // scalar registers; each round costs two 32x32->64 multiplies (v_mad_u64_u32)
-// and two 3-input XORs per lane.
+// and two 3-input XORs per lane.  Where a round's XOR has two operands that
+// depend only on uniform counter parts (block index, stream) and the key --
+// words 0 and 2 of round 0, word 2 of round 1 and word 0 of round 2 (counting
+// from 0) -- it is written as plain XORs, so that the compiler keeps those
+// words, and their multiplies, in scalar registers; v_bitop3_b32 would pull
+// them into the vector unit.
 __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                        uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)kM0 * c0;
         const uint64_t p1 = (uint64_t)kM1 * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        const uint32_t h1 = (uint32_t)(p1 >> 32), h0 = (uint32_t)(p0 >> 32);
+        const uint32_t n0 = r < 3 ? h1 ^ c1 ^ k0 : xor3(h1, c1, k0);
+        const uint32_t n2 = r < 2 ? h0 ^ c3 ^ k1 : xor3(h0, c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -103,7 +109,8 @@ __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
 //     x' = nx·z + x·nz - y·q,  y' = ny·z + y·nz + x·q,  z' = nz·z - (x·nx + y·ny).
 // For z < 0, 1+z is taken as (x²+y²)/(1-z), which stays consistent with x, y
 // near the south pole.  At the exact pole (x = y = 0, z < 0) the result is the
-// reference's (nx, -ny, -nz).  Unit inputs give |r|² = 1 + e with |e| of a few
+// reference's (nx, -ny, -nz): x' and y' are set (q is 0/0 there), z' is
+// already nz·z = -nz.  Unit inputs give |r|² = 1 + e with |e| of a few
 // 1e-6 (rounding and the hardware sin/cos), so the renormalising 1/sqrt(|r|²)
 // is one Newton step from 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an
 // FP32 ulp; it needs no transcendental.
@@ -123,7 +130,6 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float 
     const bool pole = (s2 == 0.0f) & !up;
     rx = pole ? nx : rx;
     ry = pole ? -ny : ry;
-    rz = pole ? -nz : rz;
     const float n2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
     const float inv = __builtin_fmaf(-0.5f, n2, 1.5f);
     ox = rx * inv;

@@ -212,11 +212,12 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         bool alive = active;
         bool hit = false;
         // steps of this photon: max_steps unless it hits at step k (k + 1)
-        uint32_t nst = active ? max_steps : 0u;
+        uint32_t nst = 0u;
 
         // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
 #define PP_STEP(K, AS, AC, PH)                                                           \
     if (alive) {                                                                        \
+        ++nst;                                                                          \
         float s = -__builtin_amdgcn_logf(2.0f - (AS)) * sca_ln2;                        \
         const float a = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));        \
         const float pp = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));       \
@@ -233,8 +234,8 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         qy = __builtin_fmaf(s, dy, qy);                                                 \
         qz = __builtin_fmaf(s, dz, qz);                                                 \
         dist += s;                                                                      \
+        __asm__ volatile("" : "+v"(qx), "+v"(qy), "+v"(qz), "+v"(dist));               \
         if (h) {                                                                        \
-            nst = (K) + 1u;                                                             \
             hit = true;                                                                 \
             alive = false;                                                              \
         } else {                                                                        \

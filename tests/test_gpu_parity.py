@@ -54,18 +54,21 @@ def test_device_philox_matches_known_answers(ctx):
 
 def test_device_rotation_kats_and_rodrigues(ctx):
     ey = [0.0, 1.0, 0.0]
-    old = np.array([[0, 0, 1], [0, 0, -1], ey, [0.2, 0.3, 1.0]], dtype=np.float32)
+    old = np.array([[0, 0, 1], [0, 0, -1], ey, [0.2, 0.3, 1.0], [0, 0, -1]], dtype=np.float32)
     old[3] /= np.linalg.norm(old[3])
-    new = np.array([ey, ey, ey, [-5.0, 1.0, -20.0]], dtype=np.float32)
+    new = np.array([ey, ey, ey, [-5.0, 1.0, -20.0], [0.48, -0.6, 0.64]], dtype=np.float32)
     new[3] /= np.linalg.norm(new[3])
     out = np.zeros_like(old)
     pp._abi.check(ctx.lib.pp_selftest_rotate(ctx.handle, pp._abi.ptr(old), pp._abi.ptr(new),
-                                             pp._abi.ptr(out), 4))
+                                             pp._abi.ptr(out), 5))
     # src/tests/test_photon_prob.jl:26-30, :37-41, :43-53
     np.testing.assert_allclose(out[0], ey, atol=1e-7)
     np.testing.assert_allclose(out[1], [0, -1, 0], atol=1e-7)
     np.testing.assert_allclose(out[2], [0, 0, -1], atol=1e-6)
     assert abs(np.linalg.norm(out[3]) - 1) < 1e-6
+    # the exact south pole with every component of the local direction nonzero:
+    # the reference's (x, -y, -z) (src/photon_prop_cuda.jl:185-189)
+    np.testing.assert_allclose(out[4], [0.48, 0.6, -0.64], atol=2e-7)
     # random pairs against Rodrigues (:55-64) and the FP64 oracle
     rng = np.random.default_rng(4)
     n = 20000
