@@ -66,13 +66,17 @@ __device__ __forceinline__ float u01(uint32_t w) {
 //         as the mantissa under the exponent of 1.0 (one v_alignbit_b32); the
 //         matching uniform on (0, 1] is u_k = 2 - a_k, also exact;
 //   p0  = ((w0 & 0xFF) << 8 | (w1 & 0xFF))·2^-16 and p1 likewise from w2, w3:
-//         16-bit azimuths in revolutions on [0, 1) (one v_perm_b32), the unit
-//         the hardware sin/cos take.
+//         16-bit azimuths in revolutions on [0, 1), the unit the hardware
+//         sin/cos take.  They are carried as 1 + p on [1, 2) (one v_perm_b32
+//         for the 16 bits, one v_lshl_or_b32 to put them under the exponent of
+//         1.0, exact), which is the same angle, so no integer-to-float
+//         conversion is needed.
 __device__ __forceinline__ float draw_a(uint32_t w) {
     return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, w, 9u));
 }
 __device__ __forceinline__ float draw_p(uint32_t hi, uint32_t lo) {
-    return (float)__builtin_amdgcn_perm(hi, lo, 0x0C0C0400u) * 0x1p-16f;
+    const uint32_t p = __builtin_amdgcn_perm(hi, lo, 0x0C0C0400u);
+    return __uint_as_float((p << 7) | 0x3F800000u);
 }
 
 // Henyey-Greenstein cosθ by inverse CDF (src/photon_prop_cuda.jl:31-38):
@@ -94,12 +98,19 @@ __device__ __forceinline__ float hg_costheta(const HG& h, float eta) {
     return c;
 }
 
-__device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
+// The same with the isotropic case fixed at compile time (kIso = h.iso), for
+// loops that are instantiated once per case.
+template <bool kIso>
+__device__ __forceinline__ float hg_costheta_a_t(const HG& h, float a) {
     const float x = __builtin_fmaf(-2.0f, a, 3.0f);
-    if (h.iso) return x;
+    if (kIso) return x;
     const float q = h.num * __builtin_amdgcn_rcpf(__builtin_fmaf(h.g, x, 1.0f));
     const float c = h.inv2g * __builtin_fmaf(-q, q, h.opg2);
     return __builtin_amdgcn_fmed3f(c, -1.0f, 1.0f);
+}
+
+__device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
+    return h.iso ? hg_costheta_a_t<true>(h, a) : hg_costheta_a_t<false>(h, a);
 }
 
 // rotate_to_axis (src/photon_prop_cuda.jl:171-219): apply to (nx,ny,nz) the
@@ -119,7 +130,7 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float 
     const float s2 = __builtin_fmaf(y, y, x * x);
     const bool up = z >= 0.0f;
     const float w = up ? 1.0f + z : s2;
-    const float m = up ? 1.0f : 1.0f - z;
+    const float m = fmaxf(1.0f, 1.0f - z);  // 1 for z >= 0, else 1 - z
     const float inv_h = m * __builtin_amdgcn_rcpf(w);
     const float cr = __builtin_fmaf(x, ny, -(y * nx));
     const float q = cr * inv_h;

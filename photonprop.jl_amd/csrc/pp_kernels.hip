@@ -136,8 +136,10 @@ struct TileCtx {
 // step, append hits (record field `request` = req_index), and reduce the
 // tile's counts and FP64 sums in a fixed order.  The result is valid in
 // thread 0 only.  Shared by the batch kernel and the serving kernel, so a
-// request's tile partials are the same whichever kernel ran them.
-template <bool kSysStores>
+// request's tile partials are the same whichever kernel ran them.  kIso
+// (= T.hg.iso, uniform per launch) takes the isotropic-scattering test out of
+// the step loop; run_tile_any picks the instance.
+template <bool kSysStores, bool kIso>
 __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& R,
                                                 uint32_t req_index, uint64_t first,
                                                 uint32_t count, pp_hit* hits, uint64_t hits_cap,
@@ -239,7 +241,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             hit = true;                                                                 \
             alive = false;                                                              \
         } else {                                                                        \
-            const float ct = ppd::hg_costheta_a(T.hg, (AC));                            \
+            const float ct = ppd::hg_costheta_a_t<kIso>(T.hg, (AC));                    \
             const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
@@ -344,6 +346,20 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     return tp;
 }
 
+template <bool kSysStores>
+__device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevReq& R,
+                                                    uint32_t req_index, uint64_t first,
+                                                    uint32_t count, pp_hit* hits,
+                                                    uint64_t hits_cap,
+                                                    unsigned long long* hit_cursor,
+                                                    double (*s_red)[3], uint32_t (*s_redi)[2]) {
+    if (T.hg.iso)
+        return run_tile<kSysStores, true>(T, R, req_index, first, count, hits, hits_cap,
+                                          hit_cursor, s_red, s_redi);
+    return run_tile<kSysStores, false>(T, R, req_index, first, count, hits, hits_cap,
+                                       hit_cursor, s_red, s_redi);
+}
+
 // Stage the medium table and knots into LDS.
 __device__ __forceinline__ TileCtx stage_tables(float4* s_table, const float4* table,
                                                 uint32_t n_points, const float* knots,
@@ -389,8 +405,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t first = ((uint64_t)tile - P.tile_start[r]) * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
-        const TilePartial tp = run_tile<false>(T, R, r, first, count, P.hits, P.hits_cap,
-                                               P.hit_cursor, s_red, s_redi);
+        const TilePartial tp = run_tile_any<false>(T, R, r, first, count, P.hits, P.hits_cap,
+                                                   P.hit_cursor, s_red, s_redi);
         if (threadIdx.x == 0) P.partials[tile] = tp;
     }
 }
@@ -1346,8 +1362,9 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
         const uint64_t hit_cap = sd->hit_cap;
         const uint64_t seq = sd->seq;
         TilePartial* parts = P.partials + (uint64_t)slot * P.max_tiles;
-        const TilePartial tp = run_tile<true>(T, R, slot, first, count, P.pool + sd->hit_base,
-                                              hit_cap, &P.hit_cursor[slot], s_red, s_redi);
+        const TilePartial tp = run_tile_any<true>(T, R, slot, first, count,
+                                                  P.pool + sd->hit_base, hit_cap,
+                                                  &P.hit_cursor[slot], s_red, s_redi);
         // every lane's (system-scope) hit stores have completed past the barrier
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __syncthreads();
