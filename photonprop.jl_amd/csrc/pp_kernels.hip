@@ -177,7 +177,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         } else if (T.spec.kind == PP_SPEC_KNOTS) {
             lambda = ppd::knot_interp(T.knots, T.spec.n_knots, T.spec.knot_step, u_l);
         } else {
-            lambda = 1.0f / __builtin_fmaf(-u_l, T.spec.d, T.spec.a);
+            // hardware reciprocal (1 ulp) rather than the correctly rounded
+            // division sequence, here and for the cone angle below
+            lambda = __builtin_amdgcn_rcpf(__builtin_fmaf(-u_l, T.spec.d, T.spec.a));
         }
         const float4 med = ppd::table_lerp(T.table, T.n_points, (lambda - T.lmin) * T.inv_dl);
         const float n_phase = med.x;
@@ -198,10 +200,11 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                 dy = sp * st;
                 dz = ct;
             } else {
-                const float ct = R.profile == PP_EMIT_ISOTROPIC
-                                     ? __builtin_fmaf(2.0f, u_th, -1.0f)
-                                     : fminf(1.0f / (n_phase * R.beta), 1.0f);
-                const float st = sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
+                const float ct =
+                    R.profile == PP_EMIT_ISOTROPIC
+                        ? __builtin_fmaf(2.0f, u_th, -1.0f)
+                        : fminf(__builtin_amdgcn_rcpf(n_phase * R.beta), 1.0f);
+                const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
                 ppd::rotate_to_axis(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st, ct, dx,
                                     dy, dz);
             }
