@@ -83,20 +83,14 @@ __device__ __forceinline__ float draw_p(uint32_t hi, uint32_t lo) {
 // cosθ = (1+g² - ((1-g²)/(1+g(2η-1)))²)/(2g), clamped; g == 0 is isotropic.
 // The uniform comes as a = 2 - η on [1, 2), so 2η-1 = 3-2a, exact in FP32;
 // the denominator 1 + g(2η-1) is then rounded once, as in the reference form.
+// With r = 1/(1 + g(2η-1)) the same expression is C - K·r², where
+// C = (1+g²)/(2g) and K = (1-g²)²/(2g) are rounded once on the host: one
+// multiply per draw fewer than (1/(2g))·(1+g² - ((1-g²)·r)²), with the same
+// error (about 1e-6 absolute for g in [0.3, 0.99]).
 struct HG {
-    float g, num, opg2, inv2g;
+    float g, c, k;
     int iso;
 };
-
-__device__ __forceinline__ float hg_costheta(const HG& h, float eta) {
-    const float x = __builtin_fmaf(2.0f, eta, -1.0f);
-    if (h.iso) return x;
-    const float den = __builtin_fmaf(h.g, x, 1.0f);
-    const float q = h.num * __builtin_amdgcn_rcpf(den);
-    float c = h.inv2g * (h.opg2 - q * q);
-    c = fminf(fmaxf(c, -1.0f), 1.0f);
-    return c;
-}
 
 // The same with the isotropic case fixed at compile time (kIso = h.iso), for
 // loops that are instantiated once per case.
@@ -104,8 +98,9 @@ template <bool kIso>
 __device__ __forceinline__ float hg_costheta_a_t(const HG& h, float a) {
     const float x = __builtin_fmaf(-2.0f, a, 3.0f);
     if (kIso) return x;
-    const float q = h.num * __builtin_amdgcn_rcpf(__builtin_fmaf(h.g, x, 1.0f));
-    const float c = h.inv2g * __builtin_fmaf(-q, q, h.opg2);
+    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(h.g, x, 1.0f));
+    // (K·r)·r: each instruction reads one scalar operand (the VALU limit)
+    const float c = __builtin_fmaf(-(h.k * r), r, h.c);
     return __builtin_amdgcn_fmed3f(c, -1.0f, 1.0f);
 }
 

@@ -798,9 +798,9 @@ ppd::HG make_hg(float g) {
     ppd::HG h;
     h.g = g;
     h.iso = g == 0.0f;
-    h.num = std::fma(-g, g, 1.0f);
-    h.opg2 = std::fma(g, g, 1.0f);
-    h.inv2g = h.iso ? 0.0f : 1.0f / (2.0f * g);
+    const double gd = g, num = 1.0 - gd * gd;
+    h.c = h.iso ? 0.0f : (float)((1.0 + gd * gd) / (2.0 * gd));
+    h.k = h.iso ? 0.0f : (float)(num * num / (2.0 * gd));
     return h;
 }
 
