@@ -236,6 +236,22 @@ class ConstMedium(pp.medium.MediumProperties):
         return np.full_like(np.asarray(w, dtype=float), self.s)
 
 
+def test_isotropic_scattering_parity():
+    # g = 0: isotropic scattering, a separate instance of the step loop;
+    # per photon against the oracle in a medium that scatters every ~2 m.
+    with pp.Context(ConstMedium(n=1.34, abs_len=20.0, sca_len=2.0, g=0.0)) as c:
+        src = cone_source(300_000, spectrum=pp.Monochromatic(450.0))
+        tgt = pp.DetectionSphere((1.5, 0.0, 1.5), 0.4)
+        res = c.propagate(src, tgt, seed=6, stream=2)
+        ref, rsum = oracle_for(c, src, tgt, seed=6, stream=2)
+    cmp = compare_hits(res.hits, ref)
+    say("isotropic scattering", res.summary, cmp)
+    assert rsum["n_hits"] > 1000
+    assert_parity(cmp, rsum["n_hits"])
+    flips = cmp["only_gpu"] + cmp["only_cpu"]
+    assert abs(int(res.summary["n_steps"]) - int(rsum["n_steps"])) <= 10 * flips
+
+
 def test_device_survival_exp_minus_d_over_abs():
     # North-star unit check: survival equals exp(-d/λ_abs) to 1e-3.
     n, d, r, la = 1.34, 12.0, 0.21, 25.0
