@@ -21,6 +21,8 @@ HIPCC_FLAGS = [
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
+    # HSA runtime: result copies on an SDMA engine (serving)
+    "-lhsa-runtime64",
 ]
 
 

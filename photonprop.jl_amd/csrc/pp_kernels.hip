@@ -20,6 +20,8 @@
 //     partials in tile order.  The grouping depends only on the request, so a
 //     request's sums are bitwise the same alone or inside any batch.
 #include <hip/hip_runtime.h>
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -1220,11 +1222,16 @@ constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | 
 constexpr size_t kSrvGateOffset = 4096;  // ticket and gate words on separate pages
 constexpr int kSrvVgprReserve = 64;      // VGPRs per SIMD lane left for other kernels
 constexpr int kSrvMaxWavesPerSimd = 4;   // of 8 wave slots per SIMD: 4 left for other kernels
+constexpr uint64_t kSrvStageRecords = 1u << 16;  // 4 MiB per half of the result staging
+constexpr uint64_t kSrvChunkTiles = 64;     // smallest range entry of a large request
+constexpr uint64_t kSrvMaxChunks = 128;     // range entries per large request at most
+constexpr uint64_t kSrvSmallTiles = 512;    // requests up to this many tiles are small
+constexpr uint64_t kSrvBacklogGrids = 1;    // published, unclaimed tiles kept: grids' worth
 
-struct SrvEntry {
+struct SrvEntry {       // global tiles [t0, t1) are the request's tiles local0..
     uint64_t t0, t1;
     uint32_t slot, pad;
-    uint64_t pad2;
+    uint64_t local0;
 };
 
 struct SrvSlot {       // written by the host before the tiles are published
@@ -1246,7 +1253,8 @@ struct SrvControl {
     uint64_t n_entries;   // range entries published
     uint64_t first_live;  // oldest entry that can still be searched
     uint64_t stop;
-    uint64_t pad[4];
+    uint64_t claimed;     // a recently claimed tile (every 64th; device-written)
+    uint64_t pad[3];
 };
 
 struct SrvParams {
@@ -1278,7 +1286,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
     extern __shared__ float4 s_table[];
     __shared__ unsigned long long s_tile;
     __shared__ SrvSlot s_desc;
-    __shared__ uint64_t s_entry_t0;
+    __shared__ uint64_t s_entry_t0, s_entry_local0;
     __shared__ uint32_t s_slot;
     __shared__ uint32_t s_last;
     __shared__ double s_red[kWaves][3];
@@ -1323,6 +1331,10 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
                     for (uint32_t k = 0; k < reps; ++k) __builtin_amdgcn_s_sleep(127);
                 }
             }
+            // how far the claims have got, for the host's publishing pace
+            if (got != kSrvExit && (got & 63u) == 0)
+                __hip_atomic_store(&P.ctl->claimed, (uint64_t)got, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
             s_tile = got;
         }
         __syncthreads();
@@ -1346,6 +1358,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
             if (tid == 0) {
                 const SrvEntry* e = &P.entries[lo % kSrvRing];
                 s_entry_t0 = sys_load(&e->t0);
+                s_entry_local0 = sys_load(&e->local0);
                 s_slot = sys_load(&e->slot);
             }
         }
@@ -1357,7 +1370,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
         __syncthreads();
         const SrvSlot* sd = &s_desc;
         const DevReq R = sd->req;
-        const uint64_t local = t - s_entry_t0;
+        const uint64_t local = t - s_entry_t0 + s_entry_local0;
         const uint64_t first = local * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
@@ -1423,6 +1436,7 @@ struct Job {
     uint32_t slot = 0;
     uint64_t hit_base = 0;
     uint64_t seq = 0;
+    uint64_t next_local = 0;  // tiles published so far
     bool done = false;
     pp_summary summary;
     std::chrono::steady_clock::time_point t_submit, t_done;
@@ -1452,12 +1466,24 @@ struct pp_server {
     SrvDone* done = nullptr;
     // device
     DevBuf table, knots, dctl, tiles_done, hit_cursor, partials, pool;
+    // result copies on an SDMA engine (no compute units): two pinned staging
+    // halves of kSrvStageRecords records, one completion signal each
+    bool hsa_up = false, sdma = false;
+    hsa_agent_t gpu_agent{}, cpu_agent{};
+    hsa_amd_sdma_engine_id_t engine{};
+    hsa_signal_t csig[2] = {};
+    pp_hit* stage = nullptr;
+    std::mutex copy_mu;
     // host bookkeeping (guarded by mu)
     std::mutex mu;
     std::condition_variable cv;
     std::thread worker;
     bool shutdown = false;
     std::deque<uint64_t> pending;
+    std::deque<uint64_t> small_q;             // admitted small jobs not yet published
+    std::deque<uint64_t> large_q;             // admitted large jobs with tiles left (FIFO)
+    uint64_t claimed_seen = 0;                // highest claimed tile seen
+    std::chrono::steady_clock::time_point claim_change;
     std::map<uint64_t, Job> jobs;
     std::vector<uint32_t> free_slots;
     std::vector<uint64_t> slot_ticket;
@@ -1539,12 +1565,78 @@ int launch_serve_kernel(pp_server* s) {
     return PP_OK;
 }
 
-// Admit pending jobs (FIFO) while a slot, pool room and ring room exist.
+// Append one range entry: tiles local0 .. local0 + c of job j.
+void publish_entry(pp_server* s, Job& j, uint64_t c) {
+    const uint64_t e = s->n_entries;
+    SrvEntry en;
+    memset(&en, 0, sizeof en);
+    en.t0 = s->published;
+    en.t1 = s->published + c;
+    en.slot = j.slot;
+    en.local0 = j.next_local;
+    memcpy(&s->entries[e % kSrvRing], &en, sizeof en);
+    s->entry_ticket[e % kSrvRing] = j.ticket;
+    s->published = en.t1;
+    s->n_entries = e + 1;
+    j.next_local += c;
+}
+
+// Publish admitted jobs' tiles.  Small jobs (up to kSrvSmallTiles tiles) go
+// out whole as soon as they are admitted, ahead of everything else.  Large
+// jobs go out first come first served, in chunks, while fewer than
+// kSrvBacklogGrids grids' worth of published tiles wait to be claimed.  So a small request waits for
+// at most that backlog, not for the tiles of the large requests before it,
+// and large requests keep their arrival order.  Each request's result does not
+// depend on the interleaving.
+//
+// Range entries retire in order (first_live stops at the oldest unfinished
+// job).  Once the ring is half full only that job may publish: it is either
+// fully published or the large job being published, needs at most
+// kSrvMaxChunks entries, and so always finishes, which frees the ring.
+void publish_locked(pp_server* s) {
+    if (s->stopping) return;  // tickets restart at resume_tile on the next launch
+    // Claim reports are plain stores from many CUs and may land out of order;
+    // the highest seen is kept.  Should the estimate stay full with no new
+    // report for 1 ms (an older report landing last), everything published is
+    // taken as claimed: that can only publish early, never stall the kernel.
+    const auto now = std::chrono::steady_clock::now();
+    const uint64_t target = std::max<uint64_t>(kSrvBacklogGrids * (uint64_t)s->grid, 4 * kSrvChunkTiles);
+    const uint64_t seen = __atomic_load_n(&s->ctl->claimed, __ATOMIC_ACQUIRE);
+    if (seen > s->claimed_seen && seen <= s->published) {
+        s->claimed_seen = seen;
+        s->claim_change = now;
+    } else if (s->published - s->claimed_seen >= target &&
+               now - s->claim_change > std::chrono::milliseconds(1)) {
+        s->claimed_seen = s->published;
+        s->claim_change = now;
+    }
+    const uint64_t n_before = s->n_entries;
+    auto in_use = [s] { return s->n_entries - s->first_live; };
+    const uint64_t pin = s->first_live < s->n_entries ? s->entry_ticket[s->first_live % kSrvRing] : 0;
+    while (!s->small_q.empty() && in_use() < kSrvRing / 2) {
+        Job& j = s->jobs[s->small_q.front()];
+        s->small_q.pop_front();
+        publish_entry(s, j, j.n_tiles);
+    }
+    while (!s->large_q.empty() && s->published - s->claimed_seen < target) {
+        Job& j = s->jobs[s->large_q.front()];
+        if (in_use() >= kSrvRing / 2 && j.ticket != pin && in_use() > 0) break;
+        if (in_use() >= kSrvRing) break;
+        const uint64_t chunk = std::max(kSrvChunkTiles, (j.n_tiles + kSrvMaxChunks - 1) / kSrvMaxChunks);
+        publish_entry(s, j, std::min(chunk, j.n_tiles - j.next_local));
+        if (j.next_local == j.n_tiles) s->large_q.pop_front();
+    }
+    if (s->n_entries != n_before) {
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+        __atomic_store_n(&s->ctl->n_entries, s->n_entries, __ATOMIC_RELEASE);
+        __atomic_store_n(&s->ctl->published, s->published, __ATOMIC_RELEASE);
+    }
+}
+
+// Admit pending jobs (FIFO) while a slot and pool room exist, then publish.
 void admit_locked(pp_server* s) {
     if (s->stopping) return;  // tickets restart at resume_tile on the next launch
-    bool published_any = false;
-    while (!s->pending.empty() && !s->free_slots.empty() &&
-           s->n_entries - s->first_live < kSrvRing) {
+    while (!s->pending.empty() && !s->free_slots.empty()) {
         Job& j = s->jobs[s->pending.front()];
         uint64_t off = 0;
         if (!pool_alloc(s, j.hits_cap, &off)) break;
@@ -1570,23 +1662,10 @@ void admit_locked(pp_server* s) {
         sd.hit_cap = j.hits_cap;
         sd.seq = j.seq;
         memcpy(&s->slots[j.slot], &sd, sizeof sd);
-        const uint64_t e = s->n_entries;
-        SrvEntry en;
-        memset(&en, 0, sizeof en);
-        en.t0 = s->published;
-        en.t1 = s->published + j.n_tiles;
-        en.slot = j.slot;
-        memcpy(&s->entries[e % kSrvRing], &en, sizeof en);
-        s->entry_ticket[e % kSrvRing] = j.ticket;
-        s->published = en.t1;
-        s->n_entries = e + 1;
-        published_any = true;
+        j.next_local = 0;
+        (j.n_tiles <= kSrvSmallTiles ? s->small_q : s->large_q).push_back(j.ticket);
     }
-    if (published_any) {
-        std::atomic_thread_fence(std::memory_order_seq_cst);
-        __atomic_store_n(&s->ctl->n_entries, s->n_entries, __ATOMIC_RELEASE);
-        __atomic_store_n(&s->ctl->published, s->published, __ATOMIC_RELEASE);
-    }
+    publish_locked(s);
 }
 
 // Completions posted by the device; advance first_live past finished entries.
@@ -1635,7 +1714,11 @@ void worker_main(pp_server* s) {
                 admit_locked(s);
             }
         }
-        const bool busy = s->completed < s->submitted;
+        // Busy while admitted requests are in flight.  Requests that wait for
+        // a slot or pool room do not count: only collecting results frees
+        // those, and a caller blocked on another kernel could not collect
+        // while this kernel held the compute units.
+        const bool busy = s->submitted - s->completed > s->pending.size();
         const auto now = std::chrono::steady_clock::now();
         if (busy) s->idle_since = now;
         if (s->kernel_live && !s->stopping && !busy &&
@@ -1644,6 +1727,7 @@ void worker_main(pp_server* s) {
             // tickets, and the next launch resumes from here
             s->stopping = true;
             s->resume_tile = s->published;
+            s->claimed_seen = s->published;
             __atomic_store_n(&s->ctl->stop, 1ull, __ATOMIC_RELEASE);
         }
         if (busy && !s->kernel_live && s->published > s->resume_tile) {
@@ -1675,7 +1759,74 @@ void free_server(pp_server* s) {
     if (s->kdone) (void)hipEventDestroy(s->kdone);
     if (s->kstream) (void)hipStreamDestroy(s->kstream);
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
+    for (hsa_signal_t& sg : s->csig)
+        if (sg.handle) (void)hsa_signal_destroy(sg);
+    if (s->stage) (void)hipHostFree(s->stage);
+    if (s->hsa_up) (void)hsa_shut_down();
     delete s;
+}
+
+hsa_status_t find_cpu_agent(hsa_agent_t a, void* data) {
+    hsa_device_type_t t;
+    if (hsa_agent_get_info(a, HSA_AGENT_INFO_DEVICE, &t) == HSA_STATUS_SUCCESS &&
+        t == HSA_DEVICE_TYPE_CPU) {
+        *static_cast<hsa_agent_t*>(data) = a;
+        return HSA_STATUS_INFO_BREAK;
+    }
+    return HSA_STATUS_SUCCESS;
+}
+
+// Start an SDMA copy of n records from the pool to staging half h.
+bool sdma_issue(pp_server* s, int h, const pp_hit* src, uint64_t n) {
+    hsa_signal_store_screlease(s->csig[h], 1);
+    return hsa_amd_memory_async_copy_on_engine(s->stage + (uint64_t)h * kSrvStageRecords,
+                                               s->cpu_agent, src, s->gpu_agent,
+                                               n * sizeof(pp_hit), 0, nullptr, s->csig[h],
+                                               s->engine, false) == HSA_STATUS_SUCCESS;
+}
+
+// Wait for staging half h; false on a copy error or after timeout_ms.
+bool sdma_wait(pp_server* s, int h, int64_t timeout_ms) {
+    const auto t_end = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
+    for (;;) {
+        const hsa_signal_value_t v = hsa_signal_wait_scacquire(
+            s->csig[h], HSA_SIGNAL_CONDITION_LT, 1, 1000000, HSA_WAIT_STATE_ACTIVE);
+        if (v < 0) return false;
+        if (v == 0) return true;
+        if (std::chrono::steady_clock::now() > t_end) return false;
+    }
+}
+
+// Set up result copies on an SDMA engine: the HSA agents of the pool's GPU and
+// of host memory, an engine with a path between them, pinned staging and one
+// test copy.  Returns false (and leaves s->sdma false) if any step fails; the
+// server then copies with hipMemcpyAsync.
+bool setup_sdma(pp_server* s) {
+    if (hsa_init() != HSA_STATUS_SUCCESS) return false;
+    s->hsa_up = true;
+    hsa_amd_pointer_info_t pi;
+    memset(&pi, 0, sizeof pi);
+    pi.size = sizeof pi;
+    if (hsa_amd_pointer_info(s->pool.p, &pi, nullptr, nullptr, nullptr) != HSA_STATUS_SUCCESS ||
+        pi.type != HSA_EXT_POINTER_TYPE_HSA)
+        return false;
+    s->gpu_agent = pi.agentOwner;
+    if (hsa_iterate_agents(find_cpu_agent, &s->cpu_agent) != HSA_STATUS_INFO_BREAK) return false;
+    uint32_t mask = 0;
+    if (hsa_amd_memory_copy_engine_status(s->cpu_agent, s->gpu_agent, &mask) != HSA_STATUS_SUCCESS ||
+        mask == 0)
+        return false;
+    s->engine = (hsa_amd_sdma_engine_id_t)(mask & (~mask + 1));  // lowest available
+    for (hsa_signal_t& sg : s->csig)
+        if (hsa_signal_create(0, 0, nullptr, &sg) != HSA_STATUS_SUCCESS) return false;
+    if (hipHostMalloc((void**)&s->stage, 2 * kSrvStageRecords * sizeof(pp_hit), 0) != hipSuccess) {
+        s->stage = nullptr;
+        return false;
+    }
+    // one record through the engine and back to the host
+    if (!sdma_issue(s, 0, (const pp_hit*)s->pool.p, 1) || !sdma_wait(s, 0, 2000)) return false;
+    s->sdma = true;
+    return true;
 }
 
 int server_setup(pp_server* s, const pp_spectrum* spectrum) {
@@ -1723,14 +1874,18 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     PP_HIP(hipStreamSynchronize(s->kstream));
     s->lds = (size_t)s->n_points * sizeof(float4) +
              (s->spec.kind == PP_SPEC_KNOTS ? (size_t)s->spec.n_knots * sizeof(float) : 0);
-    // The persistent kernel must leave room on every CU for other kernels: the
-    // runtime's copy kernels behind pp_server_result's device-to-host copy
-    // among them.  If it filled a CU, a result copy would wait for the kernel
-    // to exit while the kernel waits (busy) for the request that the copy's
-    // pool release would admit.  Each block puts one wave on each SIMD; keep
-    // kSrvVgprReserve of the 512 VGPRs per lane and half the 8 wave slots free
-    // (a 64-VGPR build at 7 blocks per CU left 64 VGPRs but one wave slot and
-    // deadlocked this way).
+    // Result records leave the device pool on an SDMA engine where one is
+    // available (no compute units involved), else through hipMemcpyAsync,
+    // whose runtime copy kernel needs room on a CU: had the persistent kernel
+    // filled them, a result copy would wait for the kernel to exit while the
+    // kernel waits (busy) for the request that the copy's pool release would
+    // admit (7 waves per SIMD, leaving 8 or 64 VGPRs and one wave slot,
+    // deadlocked this way).  Either way the grid leaves room on every CU for
+    // other kernels: each block puts one wave on each SIMD, and kSrvVgprReserve
+    // of the 512 VGPRs per lane and half the 8 wave slots stay free.  (A
+    // 64-VGPR build filling all 8 slots served the mixed request load 20-25 %
+    // slower than this one at 4: its spills cost more than the extra waves.)
+    setup_sdma(s);
     int per_cu = 0;
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, s->lds));
     hipFuncAttributes fa;
@@ -1839,7 +1994,30 @@ int pp_server_result(pp_server* s, uint64_t ticket, pp_summary* summary, pp_hit*
         j = it->second;
     }
     const uint64_t n = std::min<uint64_t>(j.summary.n_stored, hits_cap);
-    if (n) {
+    if (n && s->sdma) {
+        // pool -> pinned staging on the SDMA engine, staging -> caller on the
+        // host, the two halves alternating so that both overlap
+        std::lock_guard<std::mutex> cg(s->copy_mu);
+        const pp_hit* src = (const pp_hit*)s->pool.p + j.hit_base;
+        uint64_t off = 0;
+        int h = 0;
+        if (!sdma_issue(s, 0, src, std::min(n, kSrvStageRecords)))
+            return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed to start",
+                        (unsigned long long)ticket);
+        while (off < n) {
+            const uint64_t m = std::min(n - off, kSrvStageRecords);
+            if (!sdma_wait(s, h, 60000))
+                return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed",
+                            (unsigned long long)ticket);
+            const uint64_t next = off + m;
+            if (next < n && !sdma_issue(s, h ^ 1, src + next, std::min(n - next, kSrvStageRecords)))
+                return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed to start",
+                            (unsigned long long)ticket);
+            memcpy(hits_out + off, s->stage + (uint64_t)h * kSrvStageRecords, m * sizeof(pp_hit));
+            off = next;
+            h ^= 1;
+        }
+    } else if (n) {
         PP_HIP(hipSetDevice(s->ctx->device));
         PP_HIP(hipMemcpyAsync(hits_out, (pp_hit*)s->pool.p + j.hit_base, n * sizeof(pp_hit),
                               hipMemcpyDeviceToHost, s->cstream));

@@ -74,6 +74,35 @@ def test_server_results_equal_alone(ctx):
         assert s["n_stored"] == h.shape[0] == min(s["n_hits"], caps[i])
 
 
+def test_server_large_requests_in_chunks_equal_alone(ctx):
+    """Requests above 512 tiles (4.2e6 photons) are published in chunks,
+    first come first served, while small ones go out whole ahead of them; a
+    request's chunks interleave with other requests' tiles.  Every result is
+    still bitwise the one it has alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = workloads.batched_pairs(8, seed=44, photons=(1e3, 1e4), distance=(6.0, 12.0))
+    reqs["n_photons"] = [6_000_000, 20_000, 7_500_000, 3_000, 5_000_000, 400_000, 9_000_000, 70_000]
+    caps = [20_000] * len(reqs)
+    say("serving: 4 large (chunked) and 4 small requests, max_slots 6")
+    with pp.Server(ctx, spec, max_slots=6, max_tiles_per_request=2048,
+                   hit_pool_records=1 << 20) as srv:
+        tickets = [srv.submit(r, caps[i]) for i, r in enumerate(reqs)]
+        got = []
+        for i, t in enumerate(tickets):
+            assert srv.wait(t, 60000), "ticket %d timed out" % i
+            got.append(srv.result(t, caps[i]))
+        st = srv.stats()
+    say("server stats", st)
+    assert st["tiles_published"] == sum((int(n) + 8191) // 8192 for n in reqs["n_photons"])
+    for i, r in enumerate(reqs):
+        s, h, _ = got[i]
+        ref = alone(ctx, r)
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (i, k, s[k], ref.summary[k])
+        assert s["n_hits"] <= caps[i], "cap too small for an exact hit comparison"
+        np.testing.assert_array_equal(key(h), key(ref.hits))
+
+
 def test_server_overflow_and_summary_only(ctx):
     spec = pp.InverseSquareSpectrum(300.0, 600.0)
     r = requests(1, seed=2)[0]
