@@ -133,9 +133,13 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float 
     float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
     float ry = __builtin_fmaf(ny, z, __builtin_fmaf(y, nz, x * q));
     float rz = __builtin_fmaf(nz, z, -dxy);
-    const bool pole = (s2 == 0.0f) & !up;
-    rx = pole ? nx : rx;
-    ry = pole ? -ny : ry;
+    if ((s2 == 0.0f) & !up) {
+        // the exact south pole: a branch almost never taken (the empty asm
+        // keeps the compiler from turning it into two selects on every call)
+        __asm__ volatile("");
+        rx = nx;
+        ry = -ny;
+    }
     const float n2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
     const float inv = __builtin_fmaf(-0.5f, n2, 1.5f);
     ox = rx * inv;
