@@ -119,7 +119,11 @@ __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
 // already nz·z = -nz.  Unit inputs give |r|² = 1 + e with |e| of a few
 // 1e-6 (rounding and the hardware sin/cos), so the renormalising 1/sqrt(|r|²)
 // is one Newton step from 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an
-// FP32 ulp; it needs no transcendental.
+// FP32 ulp; it needs no transcendental.  kRenorm = false skips it: one rotation
+// of a unit vector moves |r| by ~2e-7, so a caller may renormalise every
+// other rotation (then |e| stays below ~1e-6 and the Newton step is still
+// exact to ~1e-13).
+template <bool kRenorm = true>
 __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float nx, float ny,
                                                float nz, float& ox, float& oy, float& oz) {
     const float s2 = __builtin_fmaf(y, y, x * x);
@@ -139,6 +143,12 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float 
         __asm__ volatile("");
         rx = nx;
         ry = -ny;
+    }
+    if (!kRenorm) {
+        ox = rx;
+        oy = ry;
+        oz = rz;
+        return;
     }
     const float n2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
     const float inv = __builtin_fmaf(-0.5f, n2, 1.5f);

@@ -222,7 +222,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         uint32_t nst = 0u;
 
         // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
-#define PP_STEP(K, AS, AC, PH)                                                           \
+#define PP_STEP(K, AS, AC, PH, RN)                                                       \
     if (alive) {                                                                        \
         ++nst;                                                                          \
         float s = -__builtin_amdgcn_logf(2.0f - (AS)) * sca_ln2;                        \
@@ -250,20 +250,22 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
-            ppd::rotate_to_axis(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);        \
+            ppd::rotate_to_axis<RN>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);    \
         }                                                                               \
     }
 
         // No per-step ballot: a wave whose lanes have all hit skips the
         // (divergent) step bodies and Philox blocks on an empty exec mask.
-        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w))
+        // The direction is renormalised after step 0 and after every even
+        // step, i.e. after every other rotation.
+        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), true)
         for (uint32_t s0 = 1; s0 < max_steps; s0 += 2) {
             ppd::U4 Q;
             if (alive)
                 Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y))
+            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false)
             if (s0 + 1 >= max_steps) break;
-            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w))
+            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), true)
         }
 #undef PP_STEP
         acc_steps += nst;
