@@ -207,8 +207,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                         ? __builtin_fmaf(2.0f, u_th, -1.0f)
                         : fminf(__builtin_amdgcn_rcpf(n_phase * R.beta), 1.0f);
                 const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
-                ppd::rotate_to_axis(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st, ct, dx,
-                                    dy, dz);
+                // not renormalised here: step 0's rotation renormalises
+                ppd::rotate_to_axis<false>(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st,
+                                           ct, dx, dy, dz);
             }
         }
         const float idx = dx, idy = dy, idz = dz;
