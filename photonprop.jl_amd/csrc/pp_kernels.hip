@@ -125,6 +125,46 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+// Hit records leave a wave through a small LDS buffer: a wave stages up to
+// kHitBuf records and takes their output slots with one atomicAdd on the hit
+// cursor when the buffer is full (or the tile ends).  One atomic per round
+// with a hit, all on the same address, stalled the waves on its return: the
+// launch ran 43 % longer than with no allocation at all.
+constexpr uint32_t kHitBuf = 8;
+
+// 16 B of a record: plain, or (serving) as system-scope 32-bit stores that go
+// straight to memory, where the host copies records while the kernel runs.
+template <bool kSysStores>
+__device__ __forceinline__ void store_quad(uint4* dst, const uint4& v) {
+    if (kSysStores) {
+        uint32_t* o = reinterpret_cast<uint32_t*>(dst);
+        __hip_atomic_store(o + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(o + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(o + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(o + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+        *dst = v;
+    }
+}
+
+// The wave's n staged records (n <= kHitBuf) to slots base .. base + n - 1
+// of `hits`, base from one atomicAdd; slots at or beyond hits_cap are dropped
+// (the cursor keeps counting, so overflow stays exact).  Lane i copies 16 B.
+// Called by the whole wave (uniform control flow).
+template <bool kSysStores>
+__device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit* hits,
+                                           uint64_t hits_cap, unsigned long long* cursor,
+                                           uint32_t lane) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(cursor, (unsigned long long)n);
+    base = __shfl(base, 0, 64);
+    if (lane < 4 * n) {
+        const unsigned long long slot = base + (lane >> 2);
+        if (slot < hits_cap)
+            store_quad<kSysStores>(reinterpret_cast<uint4*>(hits + slot) + (lane & 3), buf[lane]);
+    }
+}
+
 struct TileCtx {
     const float4* table;  // LDS copy of the medium table
     const float* knots;   // LDS copy of the spectrum knots
@@ -146,10 +186,13 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                                                 uint32_t req_index, uint64_t first,
                                                 uint32_t count, pp_hit* hits, uint64_t hits_cap,
                                                 unsigned long long* hit_cursor,
-                                                double (*s_red)[3], uint32_t (*s_redi)[2]) {
+                                                double (*s_red)[3], uint32_t (*s_redi)[2],
+                                                uint4 (*s_hitbuf)[kHitBuf * 4]) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
+    uint4* const hbuf = s_hitbuf[wave];
+    uint32_t fill = 0;  // records staged in hbuf (wave-uniform)
     const uint32_t rounds = (count + kBlock - 1) / kBlock;
     const uint32_t max_steps = R.max_steps;
     const float r2 = R.r2;
@@ -271,12 +314,19 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
 #undef PP_STEP
         acc_steps += nst;
 
-        // ---- hit append: one atomic per wave with any hit ----
+        // ---- hit append: staged per wave, one slot allocation per flush ----
         const uint64_t mask = __ballot(hit);
         if (mask != 0) {
+            const uint32_t h = (uint32_t)__popcll(mask);
             unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(hit_cursor, (unsigned long long)__popcll(mask));
-            base = __shfl(base, 0, 64);
+            if (h > kHitBuf) {
+                // more hits in this round than the buffer holds: allocate them at once
+                if (lane == 0) base = atomicAdd(hit_cursor, (unsigned long long)h);
+                base = __shfl(base, 0, 64);
+            } else if (fill + h > kHitBuf) {
+                flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane);
+                fill = 0;
+            }
             if (hit) {
                 const float px = qx + cx, py = qy + cy, pz = qz + cz;
                 const float e = -dist / abs_len;
@@ -288,39 +338,34 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                 acc_wt2 += (double)wgt * (double)t * (double)t;
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                const unsigned long long slot = base + rank;
-                if (slot < hits_cap) {
-                    if (kSysStores) {
-                        // serving: the host copies records while the kernel runs,
-                        // so they go straight to memory (no L2 write-back needed)
-                        // (32-bit words: no register pairs to assemble)
-                        uint32_t* o = reinterpret_cast<uint32_t*>(hits + slot);
-                        const uint32_t v[16] = {
-                            __float_as_uint(px),  __float_as_uint(py),  __float_as_uint(pz),
-                            __float_as_uint(t),   __float_as_uint(dx),  __float_as_uint(dy),
-                            __float_as_uint(dz),  __float_as_uint(lambda),
-                            __float_as_uint(idx), __float_as_uint(idy), __float_as_uint(idz),
-                            __float_as_uint(wgt), __float_as_uint(dist), req_index, id_lo, id_hi};
-#pragma unroll
-                        for (int q = 0; q < 16; ++q)
-                            __hip_atomic_store(o + q, v[q], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_SYSTEM);
-                    } else {
-                        float4* o = reinterpret_cast<float4*>(hits + slot);
-                        o[0] = make_float4(px, py, pz, t);
-                        o[1] = make_float4(dx, dy, dz, lambda);
-                        o[2] = make_float4(idx, idy, idz, wgt);
-                        uint4 tail;
-                        tail.x = __float_as_uint(dist);
-                        tail.y = req_index;
-                        tail.z = id_lo;
-                        tail.w = id_hi;
-                        reinterpret_cast<uint4*>(o)[3] = tail;
+                const uint4 q0 = make_uint4(__float_as_uint(px), __float_as_uint(py),
+                                            __float_as_uint(pz), __float_as_uint(t));
+                const uint4 q1 = make_uint4(__float_as_uint(dx), __float_as_uint(dy),
+                                            __float_as_uint(dz), __float_as_uint(lambda));
+                const uint4 q2 = make_uint4(__float_as_uint(idx), __float_as_uint(idy),
+                                            __float_as_uint(idz), __float_as_uint(wgt));
+                const uint4 q3 = make_uint4(__float_as_uint(dist), req_index, id_lo, id_hi);
+                if (h > kHitBuf) {
+                    const unsigned long long slot = base + rank;
+                    if (slot < hits_cap) {
+                        uint4* o = reinterpret_cast<uint4*>(hits + slot);
+                        store_quad<kSysStores>(o + 0, q0);
+                        store_quad<kSysStores>(o + 1, q1);
+                        store_quad<kSysStores>(o + 2, q2);
+                        store_quad<kSysStores>(o + 3, q3);
                     }
+                } else {
+                    uint4* o = hbuf + (fill + rank) * 4;
+                    o[0] = q0;
+                    o[1] = q1;
+                    o[2] = q2;
+                    o[3] = q3;
                 }
             }
+            if (h <= kHitBuf) fill += h;
         }
     }
+    if (fill) flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane);
 
     // ---- tile partial: fixed-order reduction ----
     const uint32_t ws = wave_sum_u32(acc_steps);
@@ -360,12 +405,13 @@ __device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevR
                                                     uint32_t count, pp_hit* hits,
                                                     uint64_t hits_cap,
                                                     unsigned long long* hit_cursor,
-                                                    double (*s_red)[3], uint32_t (*s_redi)[2]) {
+                                                    double (*s_red)[3], uint32_t (*s_redi)[2],
+                                                    uint4 (*s_hitbuf)[kHitBuf * 4]) {
     if (T.hg.iso)
         return run_tile<kSysStores, true>(T, R, req_index, first, count, hits, hits_cap,
-                                          hit_cursor, s_red, s_redi);
+                                          hit_cursor, s_red, s_redi, s_hitbuf);
     return run_tile<kSysStores, false>(T, R, req_index, first, count, hits, hits_cap,
-                                       hit_cursor, s_red, s_redi);
+                                       hit_cursor, s_red, s_redi, s_hitbuf);
 }
 
 // Stage the medium table and knots into LDS.
@@ -393,6 +439,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ uint32_t s_tile;
     __shared__ double s_red[kWaves][3];
     __shared__ uint32_t s_redi[kWaves][2];
+    __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
                                    P.inv_dl, P.hg);
     for (;;) {
@@ -414,7 +461,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
         const TilePartial tp = run_tile_any<false>(T, R, r, first, count, P.hits, P.hits_cap,
-                                                   P.hit_cursor, s_red, s_redi);
+                                                   P.hit_cursor, s_red, s_redi, s_hitbuf);
         if (threadIdx.x == 0) P.partials[tile] = tp;
     }
 }
@@ -1294,6 +1341,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
     __shared__ uint32_t s_last;
     __shared__ double s_red[kWaves][3];
     __shared__ uint32_t s_redi[kWaves][2];
+    __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
                                    P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
@@ -1383,7 +1431,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
         TilePartial* parts = P.partials + (uint64_t)slot * P.max_tiles;
         const TilePartial tp = run_tile_any<true>(T, R, slot, first, count,
                                                   P.pool + sd->hit_base, hit_cap,
-                                                  &P.hit_cursor[slot], s_red, s_redi);
+                                                  &P.hit_cursor[slot], s_red, s_redi, s_hitbuf);
         // every lane's (system-scope) hit stores have completed past the barrier
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __syncthreads();

@@ -147,6 +147,38 @@ def test_server_open_loop_poisson(ctx):
     assert out["steps"] > 9 * out["photons"]
 
 
+def test_server_queued_request_does_not_hold_the_kernel(ctx):
+    """A request waiting for pool room does not keep the persistent kernel
+    running: with A holding the whole pool and done, the kernel idles out
+    (other kernels run meanwhile), collecting A's result admits B, and B runs
+    on a relaunched kernel with the same result as alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(2, seed=55)
+    reqs["n_photons"] = [100_000, 120_000]
+    pool = 1 << 16
+    with pp.Server(ctx, spec, max_slots=4, max_tiles_per_request=64, hit_pool_records=pool,
+                   idle_exit_us=2000) as srv:
+        ta = srv.submit(reqs[0], pool)
+        tb = srv.submit(reqs[1], pool)  # waits: A holds the whole pool
+        assert srv.wait(ta, 60000)
+        time.sleep(0.05)  # well past idle_exit_us
+        st = srv.stats()
+        assert st["pending"] == 1 and st["in_flight"] == 0, st
+        other = alone(ctx, reqs[0])  # another kernel on the device meanwhile
+        sa, ha, _ = srv.result(ta, pool)
+        assert srv.wait(tb, 60000)
+        sb, hb, _ = srv.result(tb, pool)
+        st = srv.stats()
+    say("queued-request stats", st)
+    assert st["kernel_launches"] >= 2
+    for k in SUM_KEYS:
+        assert sa[k] == other.summary[k], (k, sa[k], other.summary[k])
+    ref = alone(ctx, reqs[1])
+    for k in SUM_KEYS:
+        assert sb[k] == ref.summary[k], (k, sb[k], ref.summary[k])
+    np.testing.assert_array_equal(key(hb), key(ref.hits))
+
+
 def test_server_idle_stop_and_relaunch(ctx):
     """The kernel stops after idle_exit_us without work and is relaunched on
     the next submission; tickets resume at the published count, so results
