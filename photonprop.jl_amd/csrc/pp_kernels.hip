@@ -125,12 +125,15 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
-// Hit records leave a wave through a small LDS buffer: a wave stages up to
-// kHitBuf records and takes their output slots with one atomicAdd on the hit
-// cursor when the buffer is full (or the tile ends).  One atomic per round
-// with a hit, all on the same address, stalled the waves on its return: the
-// launch ran 43 % longer than with no allocation at all.
-constexpr uint32_t kHitBuf = 8;
+// Hit records leave a wave through an LDS buffer: a wave stages up to kHitBuf
+// records and takes their output slots with one atomicAdd on the hit cursor
+// when the next round's hits would not fit (or the tile ends).  One atomic per
+// round with a hit, all on the same address, stalled the waves on its return:
+// the launch ran 74 % longer than with no allocation at all (1,869 against
+// 1,074 ms); 8, 16, 32 and 64 records per wave gave 1,487, 1,257, 1,160 and
+// 1,135 ms.  64 records (16 KiB per block) also always take a whole round.
+constexpr uint32_t kHitBuf = 64;
+static_assert(kHitBuf >= 64, "a round's hits (up to 64) must fit an empty buffer");
 
 // 16 B of a record: plain, or (serving) as system-scope 32-bit stores that go
 // straight to memory, where the host copies records while the kernel runs.
@@ -149,8 +152,8 @@ __device__ __forceinline__ void store_quad(uint4* dst, const uint4& v) {
 
 // The wave's n staged records (n <= kHitBuf) to slots base .. base + n - 1
 // of `hits`, base from one atomicAdd; slots at or beyond hits_cap are dropped
-// (the cursor keeps counting, so overflow stays exact).  Lane i copies 16 B.
-// Called by the whole wave (uniform control flow).
+// (the cursor keeps counting, so overflow stays exact).  The lanes copy 16 B
+// at a time.  Called by the whole wave (uniform control flow).
 template <bool kSysStores>
 __device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit* hits,
                                            uint64_t hits_cap, unsigned long long* cursor,
@@ -158,16 +161,16 @@ __device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit*
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(cursor, (unsigned long long)n);
     base = __shfl(base, 0, 64);
-    if (lane < 4 * n) {
-        const unsigned long long slot = base + (lane >> 2);
+    for (uint32_t c = lane; c < 4 * n; c += 64) {
+        const unsigned long long slot = base + (c >> 2);
         if (slot < hits_cap)
-            store_quad<kSysStores>(reinterpret_cast<uint4*>(hits + slot) + (lane & 3), buf[lane]);
+            store_quad<kSysStores>(reinterpret_cast<uint4*>(hits + slot) + (c & 3), buf[c]);
     }
 }
 
 struct TileCtx {
-    const float4* table;  // LDS copy of the medium table
-    const float* knots;   // LDS copy of the spectrum knots
+    const float4* table;  // medium table (device memory; read once per photon, cached)
+    const float* knots;   // spectrum knots (device memory)
     uint32_t n_points;
     float lmin, inv_dl;
     ppd::HG hg;
@@ -318,12 +321,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         const uint64_t mask = __ballot(hit);
         if (mask != 0) {
             const uint32_t h = (uint32_t)__popcll(mask);
-            unsigned long long base = 0;
-            if (h > kHitBuf) {
-                // more hits in this round than the buffer holds: allocate them at once
-                if (lane == 0) base = atomicAdd(hit_cursor, (unsigned long long)h);
-                base = __shfl(base, 0, 64);
-            } else if (fill + h > kHitBuf) {
+            if (fill + h > kHitBuf) {
                 flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane);
                 fill = 0;
             }
@@ -345,24 +343,13 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                 const uint4 q2 = make_uint4(__float_as_uint(idx), __float_as_uint(idy),
                                             __float_as_uint(idz), __float_as_uint(wgt));
                 const uint4 q3 = make_uint4(__float_as_uint(dist), req_index, id_lo, id_hi);
-                if (h > kHitBuf) {
-                    const unsigned long long slot = base + rank;
-                    if (slot < hits_cap) {
-                        uint4* o = reinterpret_cast<uint4*>(hits + slot);
-                        store_quad<kSysStores>(o + 0, q0);
-                        store_quad<kSysStores>(o + 1, q1);
-                        store_quad<kSysStores>(o + 2, q2);
-                        store_quad<kSysStores>(o + 3, q3);
-                    }
-                } else {
-                    uint4* o = hbuf + (fill + rank) * 4;
-                    o[0] = q0;
-                    o[1] = q1;
-                    o[2] = q2;
-                    o[3] = q3;
-                }
+                uint4* o = hbuf + (fill + rank) * 4;
+                o[0] = q0;
+                o[1] = q1;
+                o[2] = q2;
+                o[3] = q3;
             }
-            if (h <= kHitBuf) fill += h;
+            fill += h;
         }
     }
     if (fill) flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane);
@@ -414,18 +401,13 @@ __device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevR
                                        hit_cursor, s_red, s_redi, s_hitbuf);
 }
 
-// Stage the medium table and knots into LDS.
-__device__ __forceinline__ TileCtx stage_tables(float4* s_table, const float4* table,
-                                                uint32_t n_points, const float* knots,
-                                                const SpecParams& spec, float lmin, float inv_dl,
-                                                const ppd::HG& hg) {
-    float* s_knots = reinterpret_cast<float*>(s_table + n_points);
-    for (uint32_t i = threadIdx.x; i < n_points; i += kBlock) s_table[i] = table[i];
-    if (spec.kind == PP_SPEC_KNOTS)
-        for (uint32_t i = threadIdx.x; i < spec.n_knots; i += kBlock) s_knots[i] = knots[i];
+// The per-launch constants of run_tile.
+__device__ __forceinline__ TileCtx make_tile_ctx(const float4* table, uint32_t n_points,
+                                                 const float* knots, const SpecParams& spec,
+                                                 float lmin, float inv_dl, const ppd::HG& hg) {
     TileCtx T;
-    T.table = s_table;
-    T.knots = s_knots;
+    T.table = table;
+    T.knots = knots;
     T.n_points = n_points;
     T.lmin = lmin;
     T.inv_dl = inv_dl;
@@ -435,15 +417,13 @@ __device__ __forceinline__ TileCtx stage_tables(float4* s_table, const float4* t
 }
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) propagate_kernel(KParams P) {
-    extern __shared__ float4 s_table[];  // n_points float4, then knots
     __shared__ uint32_t s_tile;
     __shared__ double s_red[kWaves][3];
     __shared__ uint32_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
-    const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
-                                   P.inv_dl, P.hg);
+    const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     for (;;) {
-        __syncthreads();  // tables staged / previous tile's s_tile and s_red consumed
+        __syncthreads();  // previous tile's s_tile and s_red consumed
         if (threadIdx.x == 0) s_tile = atomicAdd(P.tile_counter, 1u);
         __syncthreads();
         const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
@@ -856,14 +836,9 @@ ppd::HG make_hg(float g) {
     return h;
 }
 
-size_t dyn_lds_bytes(const pp_ctx* c, const SpecParams& sp) {
-    return (size_t)c->n_points * sizeof(float4) +
-           (sp.kind == PP_SPEC_KNOTS ? (size_t)sp.n_knots * sizeof(float) : 0);
-}
-
-int grid_for(pp_ctx* c, size_t lds, uint64_t n_tiles, int* grid) {
+int grid_for(pp_ctx* c, uint64_t n_tiles, int* grid) {
     int per_cu = 0;
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_kernel, kBlock, lds));
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_kernel, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
     c->blocks_per_cu = per_cu;
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)c->n_cu;
@@ -906,9 +881,8 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
     fill_u32_kernel<<<1, 64, 0, c->stream>>>((uint32_t*)c->counters.p, 16, 0u);
     PP_HIP(hipGetLastError());
 
-    const size_t lds = dyn_lds_bytes(c, sp);
     int grid = 0;
-    if ((rc = grid_for(c, lds, n_tiles, &grid))) return rc;
+    if ((rc = grid_for(c, n_tiles, &grid))) return rc;
     KParams P;
     memset(&P, 0, sizeof P);
     P.reqs = (const DevReq*)c->reqs.p;
@@ -929,7 +903,7 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
     P.partials = (TilePartial*)c->partials.p;
     PP_HIP(hipEventRecord(c->ev0, c->stream));
     if (grid > 0) {
-        propagate_kernel<<<grid, kBlock, lds, c->stream>>>(P);
+        propagate_kernel<<<grid, kBlock, 0, c->stream>>>(P);
         PP_HIP(hipGetLastError());
     }
     PP_HIP(hipEventRecord(c->ev1, c->stream));
@@ -1333,7 +1307,6 @@ __device__ __forceinline__ T sys_load(const T* p) {
 }
 
 __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
-    extern __shared__ float4 s_table[];
     __shared__ unsigned long long s_tile;
     __shared__ SrvSlot s_desc;
     __shared__ uint64_t s_entry_t0, s_entry_local0;
@@ -1342,8 +1315,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
     __shared__ double s_red[kWaves][3];
     __shared__ uint32_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
-    const TileCtx T = stage_tables(s_table, P.table, P.n_points, P.knots, P.spec, P.lmin,
-                                   P.inv_dl, P.hg);
+    const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
 
     for (;;) {
@@ -1499,7 +1471,6 @@ struct pp_server {
     pp_ctx* ctx = nullptr;
     pp_server_config cfg;
     SpecParams spec;
-    size_t lds = 0;
     int grid = 0;
     uint32_t n_points = 0;
     float lmin = 0.0f, inv_dl = 0.0f, g = 0.0f;
@@ -1608,7 +1579,7 @@ int launch_serve_kernel(pp_server* s) {
     P.spec = s->spec;
     // every tile below resume_tile is done and none above it was started
     serve_reset_kernel<<<1, 64, 0, s->kstream>>>(P.ticket, P.gate, s->resume_tile);
-    serve_kernel<<<s->grid, kBlock, s->lds, s->kstream>>>(P);
+    serve_kernel<<<s->grid, kBlock, 0, s->kstream>>>(P);
     PP_HIP(hipGetLastError());
     PP_HIP(hipEventRecord(s->kdone, s->kstream));
     s->kernel_live = true;
@@ -1923,8 +1894,6 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->hit_cursor.p, 2ull * S, 0u);
     PP_HIP(hipGetLastError());
     PP_HIP(hipStreamSynchronize(s->kstream));
-    s->lds = (size_t)s->n_points * sizeof(float4) +
-             (s->spec.kind == PP_SPEC_KNOTS ? (size_t)s->spec.n_knots * sizeof(float) : 0);
     // Result records leave the device pool on an SDMA engine where one is
     // available (no compute units involved), else through hipMemcpyAsync,
     // whose runtime copy kernel needs room on a CU: had the persistent kernel
@@ -1938,7 +1907,7 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     // slower than this one at 4: its spills cost more than the extra waves.)
     setup_sdma(s);
     int per_cu = 0;
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, s->lds));
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, 0));
     hipFuncAttributes fa;
     PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
     const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
