@@ -125,9 +125,9 @@ __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
 // 1e-6 (rounding and the hardware sin/cos), so the renormalising 1/sqrt(|r|²)
 // is one Newton step from 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an
 // FP32 ulp; it needs no transcendental.  kRenorm = false skips it: one rotation
-// of a unit vector moves |r| by ~2e-7, so a caller may renormalise every
-// other rotation (then |e| stays below ~1e-6 and the Newton step is still
-// exact to ~1e-13).
+// of a unit vector moves |r| by ~2e-7, so a caller may renormalise every few
+// rotations (after four, |e| stays below ~1e-6 and the Newton step is still
+// exact to ~1e-12).
 template <bool kRenorm = true>
 __device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float nx, float ny,
                                                float nz, float& ox, float& oy, float& oz) {

@@ -303,16 +303,23 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
 
         // No per-step ballot: a wave whose lanes have all hit skips the
         // (divergent) step bodies and Philox blocks on an empty exec mask.
-        // The direction is renormalised after step 0 and after every even
-        // step, i.e. after every other rotation.
+        // The direction is renormalised after step 0 and after steps 4, 8, ...
+        // (at most four rotations apart; the emission direction is left to
+        // step 0).  Two Philox blocks per iteration, one per pair of steps.
         PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), true)
-        for (uint32_t s0 = 1; s0 < max_steps; s0 += 2) {
+        for (uint32_t s0 = 1; s0 < max_steps; s0 += 4) {
             ppd::U4 Q;
             if (alive)
                 Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
             PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false)
             if (s0 + 1 >= max_steps) break;
-            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), true)
+            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), false)
+            if (s0 + 2 >= max_steps) break;
+            if (alive)
+                Q = ppd::philox10((s0 + 3u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(s0 + 2u, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false)
+            if (s0 + 3 >= max_steps) break;
+            PP_STEP(s0 + 3u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), true)
         }
 #undef PP_STEP
         acc_steps += nst;
