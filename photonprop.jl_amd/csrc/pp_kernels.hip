@@ -153,11 +153,30 @@ __device__ __forceinline__ void store_quad(uint4* dst, const uint4& v) {
 // The wave's n staged records (n <= kHitBuf) to slots base .. base + n - 1
 // of `hits`, base from one atomicAdd; slots at or beyond hits_cap are dropped
 // (the cursor keeps counting, so overflow stays exact).  The lanes copy 16 B
-// at a time.  Called by the whole wave (uniform control flow).
+// at a time.  The records' FP64 sums Σw, Σw·t, Σw·t² (record r on lane r, a
+// fixed-order wave sum) and their count are added to the wave's row
+// wsum[0..2], *whits, whether stored or not.  Called by the whole wave
+// (uniform control flow).
 template <bool kSysStores>
 __device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit* hits,
                                            uint64_t hits_cap, unsigned long long* cursor,
-                                           uint32_t lane) {
+                                           uint32_t lane, double* wsum, uint32_t* whits) {
+    double w = 0.0, wt = 0.0, wt2 = 0.0;
+    if (lane < n) {
+        const double t = (double)__uint_as_float(buf[4 * lane].w);
+        w = (double)__uint_as_float(buf[4 * lane + 2].w);
+        wt = w * t;
+        wt2 = wt * t;
+    }
+    w = wave_sum_f64(w);
+    wt = wave_sum_f64(wt);
+    wt2 = wave_sum_f64(wt2);
+    if (lane == 0) {
+        wsum[0] += w;
+        wsum[1] += wt;
+        wsum[2] += wt2;
+        *whits += n;
+    }
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(cursor, (unsigned long long)n);
     base = __shfl(base, 0, 64);
@@ -196,13 +215,21 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     const uint32_t wave = tid >> 6;
     uint4* const hbuf = s_hitbuf[wave];
     uint32_t fill = 0;  // records staged in hbuf (wave-uniform)
+    // The hits' count and FP64 sums accumulate per wave in s_red / s_redi
+    // (lane 0 owns its wave's row; the caller's barrier before the tile has
+    // released the previous tile's rows), added at each flush.
+    if (lane == 0) {
+        s_red[wave][0] = 0.0;
+        s_red[wave][1] = 0.0;
+        s_red[wave][2] = 0.0;
+        s_redi[wave][1] = 0u;
+    }
     const uint32_t rounds = (count + kBlock - 1) / kBlock;
     const uint32_t max_steps = R.max_steps;
     const float r2 = R.r2;
     const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
 
-    uint32_t acc_steps = 0, acc_hits = 0;
-    double acc_w = 0.0, acc_wt = 0.0, acc_wt2 = 0.0;
+    uint32_t acc_steps = 0;
 
     for (uint32_t k = 0; k < rounds; ++k) {
         const uint32_t local = k * kBlock + tid;
@@ -329,7 +356,8 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         if (mask != 0) {
             const uint32_t h = (uint32_t)__popcll(mask);
             if (fill + h > kHitBuf) {
-                flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane);
+                flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane, s_red[wave],
+                                       &s_redi[wave][1]);
                 fill = 0;
             }
             if (hit) {
@@ -337,10 +365,6 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                 const float e = -dist / abs_len;
                 const float wgt = __builtin_amdgcn_exp2f(e * kLog2e);
                 const float t = R.t0 + dist * n_phase / (float)kCVacuum;
-                acc_hits += 1;
-                acc_w += (double)wgt;
-                acc_wt += (double)wgt * (double)t;
-                acc_wt2 += (double)wgt * (double)t * (double)t;
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                 const uint4 q0 = make_uint4(__float_as_uint(px), __float_as_uint(py),
@@ -359,21 +383,13 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             fill += h;
         }
     }
-    if (fill) flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane);
+    if (fill)
+        flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane, s_red[wave],
+                               &s_redi[wave][1]);
 
     // ---- tile partial: fixed-order reduction ----
     const uint32_t ws = wave_sum_u32(acc_steps);
-    const uint32_t wh = wave_sum_u32(acc_hits);
-    const double w1 = wave_sum_f64(acc_w);
-    const double w2 = wave_sum_f64(acc_wt);
-    const double w3 = wave_sum_f64(acc_wt2);
-    if (lane == 0) {
-        s_redi[wave][0] = ws;
-        s_redi[wave][1] = wh;
-        s_red[wave][0] = w1;
-        s_red[wave][1] = w2;
-        s_red[wave][2] = w3;
-    }
+    if (lane == 0) s_redi[wave][0] = ws;
     __syncthreads();
     TilePartial tp;
     tp.n_steps = 0;
