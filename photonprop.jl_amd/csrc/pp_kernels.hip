@@ -1252,8 +1252,13 @@ int pp_selftest_hg_moments(pp_ctx* c, float g, uint64_t seed, uint64_t n, double
 // until the tile is published.  The published count is mirrored from pinned
 // memory into a device-memory gate word; only the block holding the lowest
 // unpublished ticket (the head) reads the host's counters over PCIe, so the
-// link carries one poller however many blocks wait.  A block then finds its
-// range entry by a 64-ary search, runs the tile with the same run_tile as the
+// link carries one poller however many blocks wait.  Before it moves the gate
+// the head also copies the newly published range entries into a device-memory
+// ring, so a block finds its range entry by a 64-ary search in device memory
+// (the per-tile reads over PCIe are one 16-byte-line slot descriptor: a
+// search over pinned memory, some 70 scattered reads a tile, had held the
+// serving kernel to 40 % of the batch kernel's throughput on the same work).
+// It runs the tile with the same run_tile as the
 // batch kernel, stores the tile partial, and the block that completes a
 // request's last tile adds the partials in tile order and posts the summary
 // to pinned memory.  The host stops the kernel (`stop`, raised into the gate
@@ -1266,7 +1271,7 @@ namespace {
 constexpr uint32_t kSrvRing = 16384;          // range-entry ring
 constexpr uint64_t kSrvExit = ~0ull;
 constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
-constexpr size_t kSrvGateOffset = 4096;  // ticket and gate words on separate pages
+constexpr size_t kSrvGateOffset = 4096;  // ticket, gate and mirror counts on separate pages
 constexpr int kSrvVgprReserve = 64;      // VGPRs per SIMD lane left for other kernels
 constexpr int kSrvMaxWavesPerSimd = 4;   // of 8 wave slots per SIMD: 4 left for other kernels
 constexpr uint64_t kSrvStageRecords = 1u << 16;  // 4 MiB per half of the result staging
@@ -1281,6 +1286,9 @@ struct SrvEntry {       // global tiles [t0, t1) are the request's tiles local0.
     uint64_t local0;
 };
 
+static_assert(sizeof(SrvEntry) % 8 == 0 && 64 % (sizeof(SrvEntry) / 8) == 0,
+              "a range entry is copied as whole 8-byte words, a whole number per wave");
+
 struct SrvSlot {       // written by the host before the tiles are published
     DevReq req;
     uint64_t n_tiles;
@@ -1288,6 +1296,8 @@ struct SrvSlot {       // written by the host before the tiles are published
     uint64_t hit_cap;
     uint64_t seq;       // completion tag
 };
+
+static_assert(sizeof(SrvSlot) <= 64 * 4, "one wave loads the slot descriptor, a word a lane");
 
 struct SrvDone {       // written by the device
     uint64_t words[8];  // pp_summary
@@ -1311,6 +1321,8 @@ struct SrvParams {
     SrvDone* done;
     unsigned long long* ticket;       // next ticket (global tile index)
     unsigned long long* gate;         // published tiles mirrored from ctl | kGateStop
+    unsigned long long* dmeta;        // [0] entries mirrored, [1] first_live when mirrored
+    SrvEntry* dentries;               // device copy of the range-entry ring
     uint32_t* tiles_done;             // [max_slots]
     unsigned long long* hit_cursor;   // [max_slots]
     TilePartial* partials;            // [max_slots][max_tiles]
@@ -1329,7 +1341,64 @@ __device__ __forceinline__ T sys_load(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-__global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
+// Agent-scope atomic loads and stores (sc1): coherent across the XCDs' L2s.
+template <typename T>
+__device__ __forceinline__ T agent_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void agent_store(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A value that is the same in every lane, moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Copy the host's range entries [mirrored, n_entries) into the device ring
+// (one wave; each lane moves one 8-byte word, 16 entries a pass, four passes
+// in flight).  Run by the head block before it moves the gate, so that every
+// tile below the gate has its entry in device memory and the other blocks
+// search there, not over PCIe.  The entries copied are those of unfinished
+// jobs: the host does not reuse their ring positions meanwhile.
+__device__ void mirror_entries(const SrvParams& P, uint32_t lane) {
+    const uint64_t ne = uniform_u64(sys_load(&P.ctl->n_entries));
+    const uint64_t fl = uniform_u64(sys_load(&P.ctl->first_live));
+    const uint64_t m = uniform_u64(agent_load(&P.dmeta[0]));
+    constexpr uint32_t kWords = sizeof(SrvEntry) / 8;
+    constexpr uint32_t kPerPass = 64 / kWords;
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(P.entries);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(P.dentries);
+    const uint32_t word = lane % kWords;
+    for (uint64_t e0 = m; e0 < ne; e0 += 4 * kPerPass) {
+        uint64_t v[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint64_t e = e0 + k * kPerPass + lane / kWords;
+            v[k] = e < ne ? sys_load(src + (e % kSrvRing) * kWords + word) : 0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint64_t e = e0 + k * kPerPass + lane / kWords;
+            if (e < ne) agent_store(dst + (e % kSrvRing) * kWords + word, v[k]);
+        }
+    }
+    // the entries have landed before the count that admits them to searches
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) {
+        agent_store(&P.dmeta[1], (unsigned long long)fl);
+        agent_store(&P.dmeta[0], (unsigned long long)ne);
+    }
+    // the counts have landed before the gate moves
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+}
+
+// At most 96 VGPRs (5 waves per SIMD's worth): the grid takes 4 waves per
+// SIMD and (512 - kSrvVgprReserve) / VGPRs must allow that; 113 would allow 3.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) serve_kernel(SrvParams P) {
     __shared__ unsigned long long s_tile;
     __shared__ SrvSlot s_desc;
     __shared__ uint64_t s_entry_t0, s_entry_local0;
@@ -1342,14 +1411,15 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
     const uint32_t tid = threadIdx.x;
 
     for (;;) {
-        __syncthreads();
-        if (tid == 0) {
-            const unsigned long long t = atomicAdd(P.ticket, 1ull);
-            unsigned long long got = kSrvExit;
+        __syncthreads();  // the previous tile's shared words consumed
+        if (tid < 64) {   // wave 0: ticket, gate, range entry, slot descriptor
+            const uint32_t lane = tid;
+            unsigned long long tk = 0;
+            if (lane == 0) tk = atomicAdd(P.ticket, 1ull);
+            const uint64_t t = uniform_u64(__shfl(tk, 0, 64));
+            uint64_t got = kSrvExit;
             for (;;) {
-                // agent-scope atomic load (sc1): coherent across the XCDs' L2s
-                const unsigned long long g =
-                    __hip_atomic_load(P.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t g = uniform_u64(agent_load(P.gate));
                 if (t < (g & ~kGateStop)) {
                     got = t;
                     break;
@@ -1358,15 +1428,17 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
                 if (t == g) {
                     // head of the queue: the one block that reads pinned memory.
                     // Relaxed is enough: every later read of host-written data
-                    // is itself an uncached system-scope load.
-                    const uint64_t hp = sys_load(&P.ctl->published);
+                    // is itself an uncached system-scope load, issued after
+                    // the branch on this one.
+                    const uint64_t hp = uniform_u64(sys_load(&P.ctl->published));
                     if (hp > t) {
-                        atomicMax(P.gate, (unsigned long long)hp);
+                        mirror_entries(P, lane);
+                        if (lane == 0) atomicMax(P.gate, (unsigned long long)hp);
                         got = t;
                         break;
                     }
-                    if (sys_load(&P.ctl->stop)) {
-                        atomicOr(P.gate, kGateStop);
+                    if (uniform_u64(sys_load(&P.ctl->stop))) {
+                        if (lane == 0) atomicOr(P.gate, kGateStop);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(16);
@@ -1377,43 +1449,56 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
                     for (uint32_t k = 0; k < reps; ++k) __builtin_amdgcn_s_sleep(127);
                 }
             }
-            // how far the claims have got, for the host's publishing pace
-            if (got != kSrvExit && (got & 63u) == 0)
-                __hip_atomic_store(&P.ctl->claimed, (uint64_t)got, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
-            s_tile = got;
+            if (lane == 0) {
+                // how far the claims have got, for the host's publishing pace
+                if (got != kSrvExit && (got & 63u) == 0)
+                    __hip_atomic_store(&P.ctl->claimed, got, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                s_tile = got;
+            }
+            if (got != kSrvExit) {
+                // Range entry holding tile t: 64-ary search of the device ring
+                // over [first_live, mirrored).  The tile's own entry lies in
+                // that range and stays in place while the tile is live; an
+                // entry below it may meanwhile be overwritten by one 16384
+                // later (a larger t0, so "not ok").  Such entries form a
+                // prefix of the range, so the highest "ok" lane is still the
+                // right one; if no lane is ok, the counts are read again.
+                uint64_t lo = 0;
+                for (bool found = false; !found;) {
+                    lo = uniform_u64(agent_load(&P.dmeta[1]));
+                    uint64_t hi = uniform_u64(agent_load(&P.dmeta[0]));
+                    found = true;
+                    while (hi - lo > 1) {
+                        const uint64_t stride = (hi - lo + 63) / 64;
+                        const uint64_t idx = lo + lane * stride;
+                        const bool ok = idx < hi && agent_load(&P.dentries[idx % kSrvRing].t0) <= t;
+                        const uint64_t bits = __ballot(ok);
+                        if (bits == 0) {
+                            found = false;
+                            break;
+                        }
+                        const uint32_t l = 63u - (uint32_t)__clzll((long long)bits);
+                        lo = lo + (uint64_t)l * stride;
+                        hi = std::min(hi, lo + stride);
+                    }
+                }
+                const SrvEntry* e = &P.dentries[lo % kSrvRing];
+                const uint32_t slot = (uint32_t)uniform_u64(agent_load(&e->slot));
+                if (lane == 0) {
+                    s_entry_t0 = agent_load(&e->t0);
+                    s_entry_local0 = agent_load(&e->local0);
+                    s_slot = slot;
+                }
+                if (lane < sizeof(SrvSlot) / 4)
+                    reinterpret_cast<uint32_t*>(&s_desc)[lane] =
+                        sys_load(reinterpret_cast<const uint32_t*>(&P.slots[slot]) + lane);
+            }
         }
         __syncthreads();
         const unsigned long long t = s_tile;
         if (t == kSrvExit) break;
-
-        // Range entry holding tile t: 64-ary search over live entries (wave 0).
-        if (tid < 64) {
-            // one read, broadcast from lane 0, so the search is wave-uniform
-            uint64_t lo = __shfl(sys_load(&P.ctl->first_live), 0, 64);
-            uint64_t hi = __shfl(sys_load(&P.ctl->n_entries), 0, 64);
-            while (hi - lo > 1) {
-                const uint64_t stride = (hi - lo + 63) / 64;
-                const uint64_t idx = lo + tid * stride;
-                const bool ok = idx < hi && sys_load(&P.entries[idx % kSrvRing].t0) <= t;
-                const uint64_t m = __ballot(ok);
-                const uint32_t l = 63u - (uint32_t)__clzll((long long)m);  // lane 0 always ok
-                lo = lo + (uint64_t)l * stride;
-                hi = std::min(hi, lo + stride);
-            }
-            if (tid == 0) {
-                const SrvEntry* e = &P.entries[lo % kSrvRing];
-                s_entry_t0 = sys_load(&e->t0);
-                s_entry_local0 = sys_load(&e->local0);
-                s_slot = sys_load(&e->slot);
-            }
-        }
-        __syncthreads();
         const uint32_t slot = s_slot;
-        if (tid < sizeof(SrvSlot) / 4)
-            reinterpret_cast<uint32_t*>(&s_desc)[tid] =
-                sys_load(reinterpret_cast<const uint32_t*>(&P.slots[slot]) + tid);
-        __syncthreads();
         const SrvSlot* sd = &s_desc;
         const DevReq R = sd->req;
         const uint64_t local = t - s_entry_t0 + s_entry_local0;
@@ -1427,8 +1512,9 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
         const TilePartial tp = run_tile_any<true>(T, R, slot, first, count,
                                                   P.pool + sd->hit_base, hit_cap,
                                                   &P.hit_cursor[slot], s_red, s_redi, s_hitbuf);
-        // every lane's (system-scope) hit stores have completed past the barrier
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        // every wave waits for its own (system-scope) hit stores to complete
+        // before the barrier; a workgroup-scope fence would not wait for them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (tid == 0) {
             uint64_t* q = reinterpret_cast<uint64_t*>(parts + local);
@@ -1438,7 +1524,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
 #pragma unroll
             for (int k = 0; k < 5; ++k)
                 __hip_atomic_store(q + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // stores done before the count
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // stores done before the count
             const uint32_t done = atomicAdd(&P.tiles_done[slot], 1u) + 1u;
             s_last = done == n_tiles;
         }
@@ -1457,7 +1543,7 @@ __global__ void __launch_bounds__(kBlock) serve_kernel(SrvParams P) {
                                        __HIP_MEMORY_SCOPE_SYSTEM);
                 atomicExch(&P.tiles_done[slot], 0u);
                 atomicExch(&P.hit_cursor[slot], 0ull);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // summary before its tag
                 __hip_atomic_store(&P.done[slot].seq, seq, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
             }
@@ -1510,7 +1596,7 @@ struct pp_server {
     SrvSlot* slots = nullptr;
     SrvDone* done = nullptr;
     // device
-    DevBuf table, knots, dctl, tiles_done, hit_cursor, partials, pool;
+    DevBuf table, knots, dctl, dentries, tiles_done, hit_cursor, partials, pool;
     // result copies on an SDMA engine (no compute units): two pinned staging
     // halves of kSrvStageRecords records, one completion signal each
     bool hsa_up = false, sdma = false;
@@ -1588,6 +1674,8 @@ int launch_serve_kernel(pp_server* s) {
     P.done = s->done;
     P.ticket = (unsigned long long*)s->dctl.p;
     P.gate = (unsigned long long*)((char*)s->dctl.p + kSrvGateOffset);
+    P.dmeta = (unsigned long long*)((char*)s->dctl.p + 2 * kSrvGateOffset);
+    P.dentries = (SrvEntry*)s->dentries.p;
     P.tiles_done = (uint32_t*)s->tiles_done.p;
     P.hit_cursor = (unsigned long long*)s->hit_cursor.p;
     P.partials = (TilePartial*)s->partials.p;
@@ -1797,7 +1885,7 @@ void free_server(pp_server* s) {
     if (s->entries) (void)hipHostFree(s->entries);
     if (s->slots) (void)hipHostFree(s->slots);
     if (s->done) (void)hipHostFree(s->done);
-    DevBuf* bufs[] = {&s->table, &s->knots, &s->dctl, &s->tiles_done, &s->hit_cursor,
+    DevBuf* bufs[] = {&s->table, &s->knots, &s->dctl, &s->dentries, &s->tiles_done, &s->hit_cursor,
                       &s->partials, &s->pool};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1905,14 +1993,15 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     memset(s->entries, 0, kSrvRing * sizeof(SrvEntry));
     memset(s->slots, 0, S * sizeof(SrvSlot));
     memset(s->done, 0, S * sizeof(SrvDone));
-    if ((rc = ensure(s->dctl, 2 * kSrvGateOffset))) return rc;
+    if ((rc = ensure(s->dctl, 3 * kSrvGateOffset))) return rc;
+    if ((rc = ensure(s->dentries, kSrvRing * sizeof(SrvEntry)))) return rc;
     if ((rc = ensure(s->tiles_done, S * sizeof(uint32_t)))) return rc;
     if ((rc = ensure(s->hit_cursor, S * sizeof(unsigned long long)))) return rc;
     if ((rc = ensure(s->partials, (size_t)S * s->cfg.max_tiles_per_request * sizeof(TilePartial))))
         return rc;
     if ((rc = ensure(s->pool, std::max<uint64_t>(s->cfg.hit_pool_records, 1) * sizeof(pp_hit))))
         return rc;
-    fill_u32_kernel<<<8, 256, 0, s->kstream>>>((uint32_t*)s->dctl.p, 2 * kSrvGateOffset / 4, 0u);
+    fill_u32_kernel<<<8, 256, 0, s->kstream>>>((uint32_t*)s->dctl.p, 3 * kSrvGateOffset / 4, 0u);
     fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->tiles_done.p, S, 0u);
     fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->hit_cursor.p, 2ull * S, 0u);
     PP_HIP(hipGetLastError());
