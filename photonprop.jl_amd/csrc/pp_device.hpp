@@ -118,37 +118,38 @@ __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
 // divides by sinθ = sqrt(1-z²) twice; with q = (x·ny - y·nx)/(1+z) the same
 // rotation reads
 //     x' = nx·z + x·nz - y·q,  y' = ny·z + y·nz + x·q,  z' = nz·z - (x·nx + y·ny).
-// For z < 0, 1+z is taken as (x²+y²)/(1-z), which stays consistent with x, y
-// near the south pole.  At the exact pole (x = y = 0, z < 0) the result is the
-// reference's (nx, -ny, -nz): x' and y' are set (q is 0/0 there), z' is
-// already nz·z = -nz.  Unit inputs give |r|² = 1 + e with |e| of a few
-// 1e-6 (rounding and the hardware sin/cos), so the renormalising 1/sqrt(|r|²)
-// is one Newton step from 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an
-// FP32 ulp; it needs no transcendental.  kRenorm = false skips it: one rotation
-// of a unit vector moves |r| by ~2e-7, so a caller may renormalise every few
-// rotations (after four, |e| stays below ~1e-6 and the Newton step is still
-// exact to ~1e-12).
+// 1+z is taken as ((1+z)² + x² + y²)/2, equal to it for a unit vector
+// (x² + y² = (1-z)(1+z)) and free of cancellation at both poles: near z = -1
+// it follows x and y (as (x²+y²)/(1-z) would), near z = 1 it is 1+z.  For
+// |r|² = 1 + e the stand-in is off by e/2, like (x²+y²)/(1-z).  One
+// reciprocal and no select between the two forms.  (Its doubling stays a
+// multiply: the output modifier mul:2 is ignored in IEEE mode, which compute
+// kernels run in.)
+// y enters as y - 2^-60: that changes no y of magnitude above 2^-36 (the
+// subtraction rounds back to y), and at the exact south pole (x = y = 0,
+// z = -1), where the rotation is otherwise undetermined (0/0), it selects the
+// rotation by π about the x axis, so the result is the reference's
+// (nx, -ny, -nz) to rounding: x' = nx exactly, y' = -ny - 2^-60·nz,
+// z' = -nz + 2^-60·ny.  (x² + y² >= 2^-120 and |q| <= 2^61 stay in range.)
+// Unit inputs give |r|² = 1 + e with |e| of a few 1e-6 (rounding and the
+// hardware sin/cos), so the renormalising 1/sqrt(|r|²) is one Newton step from
+// 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an FP32 ulp; it needs no
+// transcendental.  kRenorm = false skips it: one rotation of a unit vector
+// moves |r| by ~2e-7, so a caller may renormalise every few rotations (after
+// four, |e| stays below ~1e-6 and the Newton step is still exact to ~1e-12).
 template <bool kRenorm = true>
-__device__ __forceinline__ void rotate_to_axis(float x, float y, float z, float nx, float ny,
+__device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, float nx, float ny,
                                                float nz, float& ox, float& oy, float& oz) {
+    const float y = y_in - 0x1p-60f;
     const float s2 = __builtin_fmaf(y, y, x * x);
-    const bool up = z >= 0.0f;
-    const float w = up ? 1.0f + z : s2;
-    const float m = fmaxf(1.0f, 1.0f - z);  // 1 for z >= 0, else 1 - z
-    const float inv_h = m * __builtin_amdgcn_rcpf(w);
+    const float zp = 1.0f + z;
+    const float inv_h = __builtin_amdgcn_rcpf(__builtin_fmaf(zp, zp, s2)) * 2.0f;
     const float cr = __builtin_fmaf(x, ny, -(y * nx));
     const float q = cr * inv_h;
     const float dxy = __builtin_fmaf(x, nx, y * ny);
-    float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
-    float ry = __builtin_fmaf(ny, z, __builtin_fmaf(y, nz, x * q));
-    float rz = __builtin_fmaf(nz, z, -dxy);
-    if ((s2 == 0.0f) & !up) {
-        // the exact south pole: a branch almost never taken (the empty asm
-        // keeps the compiler from turning it into two selects on every call)
-        __asm__ volatile("");
-        rx = nx;
-        ry = -ny;
-    }
+    const float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
+    const float ry = __builtin_fmaf(ny, z, __builtin_fmaf(y, nz, x * q));
+    const float rz = __builtin_fmaf(nz, z, -dxy);
     if (!kRenorm) {
         ox = rx;
         oy = ry;
