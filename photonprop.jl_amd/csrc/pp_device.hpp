@@ -98,19 +98,28 @@ struct HG {
 };
 
 // The same with the isotropic case fixed at compile time (kIso = h.iso), for
-// loops that are instantiated once per case.
+// loops that are instantiated once per case, and without the final clamp to
+// [-1, 1]: C - K·r² can leave the interval by an ulp or two.  The step loop
+// clamps 1 - cos² to [0, 1] instead, which costs nothing there (the clamp bit
+// of the fma that forms it), and takes the unclamped cosine as the rotation's
+// nz, an error the renormalisation absorbs.
 template <bool kIso>
 __device__ __forceinline__ float hg_costheta_a_t(const HG& h, float a) {
     const float x = __builtin_fmaf(-2.0f, a, 3.0f);
     if (kIso) return x;
     const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(h.g, x, 1.0f));
     // (K·r)·r: each instruction reads one scalar operand (the VALU limit)
-    const float c = __builtin_fmaf(-(h.k * r), r, h.c);
-    return __builtin_amdgcn_fmed3f(c, -1.0f, 1.0f);
+    return __builtin_fmaf(-(h.k * r), r, h.c);
+}
+
+// sinθ from cosθ, 1 - cos² clamped to [0, 1].
+__device__ __forceinline__ float sin_from_cos(float c) {
+    return __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(__builtin_fmaf(-c, c, 1.0f), 0.0f, 1.0f));
 }
 
 __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
-    return h.iso ? hg_costheta_a_t<true>(h, a) : hg_costheta_a_t<false>(h, a);
+    const float c = h.iso ? hg_costheta_a_t<true>(h, a) : hg_costheta_a_t<false>(h, a);
+    return __builtin_amdgcn_fmed3f(c, -1.0f, 1.0f);
 }
 
 // rotate_to_axis (src/photon_prop_cuda.jl:171-219): apply to (nx,ny,nz) the

@@ -81,6 +81,9 @@ struct pp_ctx {
     // work buffers
     pph::DevBuf reqs, tile_start, partials, counters, staging, cnt, fill, sums, offsets, out_hits;
     pph::DevBuf scratch;
+    // pinned staging for hit records going to the caller's host memory
+    void* pinned[2] = {nullptr, nullptr};
+    hipEvent_t copy_ev[2] = {nullptr, nullptr};
     std::vector<pph::DevReq> h_reqs;
     std::vector<uint64_t> h_tiles;
 };

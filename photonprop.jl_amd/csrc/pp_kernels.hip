@@ -321,7 +321,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             alive = false;                                                              \
         } else {                                                                        \
             const float ct = ppd::hg_costheta_a_t<kIso>(T.hg, (AC));                    \
-            const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));     \
+            const float st = ppd::sin_from_cos(ct);                                     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
             ppd::rotate_to_axis<RN>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);    \
@@ -1051,6 +1051,10 @@ int pp_destroy(pp_ctx* c) {
         if (b->p) (void)hipFree(b->p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int h = 0; h < 2; ++h) {
+        if (c->pinned[h]) (void)hipHostFree(c->pinned[h]);
+        if (c->copy_ev[h]) (void)hipEventDestroy(c->copy_ev[h]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PP_OK;
@@ -1062,6 +1066,48 @@ int pp_device_info(pp_ctx* c, char* name, size_t name_cap, int* n_cu) {
     if (n_cu) *n_cu = c->n_cu;
     return PP_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Hit records to the caller's host memory (pageable as a rule): in chunks
+// through two pinned staging halves, the DMA of one chunk overlapping the host
+// copy of the one before.  hipMemcpy straight into pageable memory stages
+// through the runtime's own small buffers, at a fraction of the link's rate.
+constexpr uint64_t kHostStageBytes = 16ull << 20;
+
+int copy_records_to_host(pp_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (bytes <= kHostStageBytes / 4) {
+        PP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return PP_OK;
+    }
+    for (int h = 0; h < 2; ++h) {
+        if (!c->pinned[h]) PP_HIP(hipHostMalloc(&c->pinned[h], kHostStageBytes, 0));
+        if (!c->copy_ev[h]) PP_HIP(hipEventCreateWithFlags(&c->copy_ev[h], hipEventDisableTiming));
+    }
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    PP_HIP(hipMemcpyAsync(c->pinned[0], s, std::min(kHostStageBytes, bytes), hipMemcpyDeviceToHost,
+                          c->stream));
+    PP_HIP(hipEventRecord(c->copy_ev[0], c->stream));
+    for (uint64_t off = 0, k = 0; off < bytes; off += kHostStageBytes, ++k) {
+        const int h = (int)(k & 1);
+        const uint64_t next = off + kHostStageBytes;
+        if (next < bytes) {  // the other half's previous chunk was copied out last pass
+            PP_HIP(hipMemcpyAsync(c->pinned[h ^ 1], s + next, std::min(kHostStageBytes, bytes - next),
+                                  hipMemcpyDeviceToHost, c->stream));
+            PP_HIP(hipEventRecord(c->copy_ev[h ^ 1], c->stream));
+        }
+        PP_HIP(hipEventSynchronize(c->copy_ev[h]));
+        memcpy(d + off, c->pinned[h], std::min(kHostStageBytes, bytes - off));
+    }
+    return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int pp_propagate(pp_ctx* c, const pp_emitter* emitter, const pp_spectrum* spectrum,
                  const pp_sphere* target, uint64_t seed, uint32_t stream, uint32_t max_steps,
@@ -1087,8 +1133,8 @@ int pp_propagate(pp_ctx* c, const pp_emitter* emitter, const pp_spectrum* spectr
                           hipMemcpyDeviceToHost, c->stream));
     PP_HIP(hipStreamSynchronize(c->stream));
     const uint64_t stored = std::min<uint64_t>(cursor, hits_cap);
-    if (stored)
-        PP_HIP(hipMemcpy(hits_out, c->staging.p, stored * sizeof(pp_hit), hipMemcpyDeviceToHost));
+    if (stored && (rc = copy_records_to_host(c, hits_out, c->staging.p, stored * sizeof(pp_hit))))
+        return rc;
     s.n_stored = stored;
     s.status = s.n_hits > stored ? PP_ERR_OVERFLOW : PP_OK;
     if (summary_out) *summary_out = s;
@@ -1120,8 +1166,8 @@ int pp_propagate_batch(pp_ctx* c, const pp_request* requests, uint32_t n_req,
                           hipMemcpyDeviceToHost, c->stream));
     PP_HIP(hipStreamSynchronize(c->stream));
     const uint64_t stored = offsets_out[n_req];
-    if (stored)
-        PP_HIP(hipMemcpy(hits_out, c->out_hits.p, stored * sizeof(pp_hit), hipMemcpyDeviceToHost));
+    if (stored && (rc = copy_records_to_host(c, hits_out, c->out_hits.p, stored * sizeof(pp_hit))))
+        return rc;
     for (uint32_t i = 0; i < n_req; ++i)
         if (summaries_out[i].status == PP_ERR_OVERFLOW)
             return fail(PP_ERR_OVERFLOW, "hit buffer overflow (capacity %llu)",

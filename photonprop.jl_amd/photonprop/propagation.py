@@ -162,7 +162,7 @@ class Context:
         del keep
         if not (rc == _abi.PP_ERR_OVERFLOW and allow_overflow):
             _abi.check(rc, self.lib)
-        return PropagationResult(hits[: summ.n_stored].copy(), summary_dict(summ))
+        return PropagationResult(hits[: summ.n_stored], summary_dict(summ))
 
     def propagate_batch(self, requests, spectrum, hits_cap=None, allow_overflow=False,
                         per_request_cap=0):
@@ -186,7 +186,7 @@ class Context:
         del keep
         if not (rc == _abi.PP_ERR_OVERFLOW and allow_overflow):
             _abi.check(rc, self.lib)
-        return hits[: int(offsets[-1])].copy(), offsets, sums
+        return hits[: int(offsets[-1])], offsets, sums
 
     def last_kernel_ms(self):
         ms = ctypes.c_float()
