@@ -301,8 +301,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         ++nst;                                                                          \
         float s = -__builtin_amdgcn_logf(2.0f - (AS)) * sca_ln2;                        \
         const float a = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));        \
-        const float pp = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));       \
-        const float b = __builtin_fmaf(a, a, r2 - pp);                                  \
+        /* |q|² - r² in one chain (r² folded into the first fma) */                     \
+        const float pmr = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, __builtin_fmaf(qx, qx, -r2))); \
+        const float b = __builtin_fmaf(a, a, -pmr);                                     \
         bool h = false;                                                                 \
         if (b >= 0.0f) {                                                                \
             const float d = -a - __builtin_amdgcn_sqrtf(b);                             \
