@@ -88,6 +88,24 @@ def test_device_rotation_kats_and_rodrigues(ctx):
     # the angle to the old direction is the local polar angle, everywhere
     cosang = np.einsum("ij,ij->i", out.astype(np.float64), a32.astype(np.float64))
     assert np.abs(cosang - v32[:, 2]).max() < 5e-6
+    # old directions within 1e-7 .. 1e-1 rad of either pole: the rotation's
+    # 1/(1+z) is taken as 2/((1+z)² + x² + y²), which must stay accurate there
+    m = 4000
+    t = 10.0 ** rng.uniform(-7, -1, m)
+    ph = rng.uniform(0.0, 2.0 * np.pi, m)
+    sgn = np.where(rng.uniform(size=m) < 0.5, -1.0, 1.0)
+    a = np.stack([t * np.cos(ph), t * np.sin(ph), sgn * np.sqrt(1.0 - t * t)], -1)
+    v = rng.normal(size=(m, 3)); v /= np.linalg.norm(v, axis=1, keepdims=True)
+    a32, v32 = a.astype(np.float32), v.astype(np.float32)
+    out = np.zeros_like(a32)
+    pp._abi.check(ctx.lib.pp_selftest_rotate(ctx.handle, pp._abi.ptr(a32), pp._abi.ptr(v32),
+                                             pp._abi.ptr(out), m))
+    worst = 0.0
+    for i in range(0, m, 7):
+        ai = a32[i].astype(np.float64); ai /= np.linalg.norm(ai)
+        rr = oracle.rodrigues_rotation(ez, ai, v32[i].astype(np.float64))
+        worst = max(worst, float(np.abs(out[i] - rr).max()))
+    assert worst < 5e-6, worst
 
 
 @pytest.mark.parametrize("g", [0.92, 0.97, 0.5])
@@ -124,6 +142,8 @@ CASES = [
     ("cone_near", dict(n=300_000), (1.5, 0.0, 1.5), 0.3, 10, 1, 0),
     ("cone_10m_90deg", dict(n=2_000_000), (10.0, 0.0, 0.0), 0.21, 10, 0, 0),
     ("cone_axis_tilted", dict(n=400_000, axis=(0.5, -0.3, 0.81)), (2.0, 0.5, 2.0), 0.5, 10, 5, 3),
+    # emitter axis at the exact south pole: the emission rotation's undetermined case
+    ("cone_axis_down", dict(n=300_000, axis=(0.0, 0.0, -1.0)), (1.5, 0.3, -1.5), 0.3, 10, 3, 0),
     ("mono_many_steps", dict(n=200_000, spectrum=pp.Monochromatic(420.0)), (0.0, 3.0, 0.5), 0.5, 37, 9, 1),
     ("uniform_sphere", dict(n=300_000, profile="UniformSphere"), (-2.0, 1.0, 0.0), 0.5, 10, 2, 0),
     ("ref_isotropic", dict(n=300_000, profile="IsotropicEmission"), (0.0, 0.0, 3.0), 0.4, 10, 0, 0),
