@@ -80,6 +80,7 @@ struct pp_ctx {
     pph::DevBuf knots;
     // work buffers
     pph::DevBuf reqs, tile_start, partials, counters, staging, cnt, fill, sums, offsets, out_hits;
+    pph::DevBuf rec_req;  // request of each staged record, written by the count pass
     pph::DevBuf scratch;
     // pinned staging for hit records going to the caller's host memory
     void* pinned[2] = {nullptr, nullptr};

@@ -544,7 +544,9 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
 // histogram counts records per request, so a block issues one global atomic
 // per request it saw instead of one per record (the staged order follows the
 // tiles, so a chunk mostly holds one or two requests).  Requests beyond
-// kLdsHistMax fall back to per-record global atomics.
+// kLdsHistMax fall back to per-record global atomics.  The count pass also
+// writes each record's request to a 4-byte array, so the scatter's own
+// histogram pass reads 4 bytes per record instead of a 64-byte record line.
 constexpr uint32_t kLdsHistMax = 4096;
 
 __device__ __forceinline__ uint32_t staged_request(const pp_hit* staged, uint64_t i) {
@@ -553,7 +555,8 @@ __device__ __forceinline__ uint32_t staged_request(const pp_hit* staged, uint64_
 
 __global__ void __launch_bounds__(256) count_kernel(const pp_hit* staged,
                                                     const unsigned long long* cursor,
-                                                    uint64_t cap, uint32_t n_req, uint32_t* cnt) {
+                                                    uint64_t cap, uint32_t n_req, uint32_t* cnt,
+                                                    uint32_t* rec_req) {
     __shared__ uint32_t hist[kLdsHistMax];
     const uint64_t n = *cursor < cap ? *cursor : cap;
     const bool lds = n_req <= kLdsHistMax;
@@ -565,6 +568,7 @@ __global__ void __launch_bounds__(256) count_kernel(const pp_hit* staged,
     __syncthreads();
     for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const uint32_t r = staged_request(staged, i);
+        rec_req[i] = r;
         if (lds) atomicAdd(&hist[r], 1u);
         else atomicAdd(&cnt[r], 1u);
     }
@@ -610,6 +614,7 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t* cnt, uint32_
 }
 
 __global__ void __launch_bounds__(256) scatter_kernel(const pp_hit* staged,
+                                                      const uint32_t* rec_req,
                                                       const unsigned long long* cursor,
                                                       uint64_t cap, uint32_t n_req,
                                                       uint64_t per_cap, const uint64_t* offsets,
@@ -624,8 +629,7 @@ __global__ void __launch_bounds__(256) scatter_kernel(const pp_hit* staged,
     if (lds) {
         for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x) hist[r] = 0;
         __syncthreads();
-        for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x)
-            atomicAdd(&hist[staged_request(staged, i)], 1u);
+        for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&hist[rec_req[i]], 1u);
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < n_req; r += blockDim.x) {
             const uint32_t h = hist[r];
@@ -946,17 +950,19 @@ int enqueue_compaction(pp_ctx* c, uint32_t n_req, const pp_hit* d_staging, uint6
     int rc;
     if ((rc = ensure(c->cnt, n_req * sizeof(uint32_t)))) return rc;
     if ((rc = ensure(c->fill, n_req * sizeof(uint32_t)))) return rc;
+    if ((rc = ensure(c->rec_req, std::max<uint64_t>(hits_cap, 1) * sizeof(uint32_t)))) return rc;
     const int zb = (int)std::min<uint64_t>(1024, (n_req + 255) / 256);
     fill_u32_kernel<<<zb, 256, 0, c->stream>>>((uint32_t*)c->cnt.p, n_req, 0u);
     fill_u32_kernel<<<zb, 256, 0, c->stream>>>((uint32_t*)c->fill.p, n_req, 0u);
     const unsigned long long* cursor = (const unsigned long long*)((uint32_t*)c->counters.p + 2);
     const int gb = 2048;
     count_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req,
-                                            (uint32_t*)c->cnt.p);
+                                            (uint32_t*)c->cnt.p, (uint32_t*)c->rec_req.p);
     scan_kernel<<<1, 1024, 0, c->stream>>>((const uint32_t*)c->cnt.p, n_req, per_cap, d_offsets,
                                            (pp_summary*)c->sums.p);
-    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req, per_cap,
-                                              d_offsets, (uint32_t*)c->fill.p, d_out);
+    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, (const uint32_t*)c->rec_req.p, cursor,
+                                              hits_cap, n_req, per_cap, d_offsets,
+                                              (uint32_t*)c->fill.p, d_out);
     PP_HIP(hipGetLastError());
     return PP_OK;
 }
@@ -1047,7 +1053,7 @@ int pp_destroy(pp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->table, &c->knots, &c->reqs, &c->tile_start, &c->partials,
                       &c->counters, &c->staging, &c->cnt, &c->fill, &c->sums,
-                      &c->offsets, &c->out_hits, &c->scratch};
+                      &c->offsets, &c->out_hits, &c->scratch, &c->rec_req};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
