@@ -150,12 +150,15 @@ def summarize(res, n_req):
 
 
 MIXED_RATES = (500.0, 1000.0, 1500.0, 2000.0, 2200.0, 2400.0, 2600.0, 2800.0, 5000.0, 20000.0)
-SMALL_RATES = (2000.0, 5000.0, 10000.0, 20000.0, 40000.0, 60000.0, 80000.0)
+SMALL_RATES = (2000.0, 5000.0, 10000.0, 20000.0, 40000.0, 60000.0, 80000.0, 120000.0, 160000.0,
+               240000.0, 400000.0, 800000.0)
 
 
 def _serving_sweep(pp, ctx, spectrum, n_req, rates, photons, label):
-    """Open-loop Poisson sweep of one request mix through (a) the server and
-    (b) one pp_propagate launch per request (FIFO).  Latency is measured from
+    """Open-loop Poisson sweep of one request mix through (a) the server,
+    driven by the native load generator (libpploadgen.so: submitting and
+    collecting threads in C++), and (b) one pp_propagate launch per request
+    (FIFO, one C-ABI call per request from Python).  Latency is measured from
     the scheduled arrival in both.  A rate is sustained when p99 < 20 ms and at
     least 95 % of the offered rate is delivered."""
     from photonprop import workloads
@@ -173,8 +176,8 @@ def _serving_sweep(pp, ctx, spectrum, n_req, rates, photons, label):
                    hit_pool_records=1 << 20) as srv:
         for rate in rates:
             arr, reqs = workloads.serving_requests(n_req, rate, seed=11, photons=photons)
-            out["server"][str(int(rate))] = summarize(pp.run_open_loop(srv, arr, reqs, hits_cap=0),
-                                                     n_req)
+            out["server"][str(int(rate))] = summarize(
+                pp.run_open_loop_native(srv, arr, reqs, hits_cap=0), n_req)
             print("%s server %d/s: %s" % (label, rate, out["server"][str(int(rate))]),
                   file=sys.stderr, flush=True)
         out["server_stats"] = srv.stats()

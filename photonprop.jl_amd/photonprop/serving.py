@@ -125,3 +125,59 @@ def run_open_loop(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
         "photons": int(np.asarray(requests["n_photons"], dtype=np.uint64).sum()),
         "steps": int(steps.sum()),
     }
+
+
+_LOADGEN = None
+LOADGEN_TIMEOUT = -100
+LOADGEN_BAD_ARG = -101
+
+
+def _loadgen():
+    """libpploadgen.so (built beside libphotonprop.so; raises if missing)."""
+    global _LOADGEN
+    if _LOADGEN is None:
+        import os
+        path = os.path.join(os.path.dirname(_abi.LIB_PATH), "libpploadgen.so")
+        if not os.path.exists(path):
+            raise OSError("libpploadgen.so not found at %s; run __graft_entry__.build()" % path)
+        lg = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        lg.ppl_open_loop.restype = ctypes.c_int
+        lg.ppl_open_loop.argtypes = [vp, vp, vp, vp, vp, dp, ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.c_double, dp, dp, ctypes.POINTER(ctypes.c_uint64), dp]
+        _LOADGEN = lg
+    return _LOADGEN
+
+
+def run_open_loop_native(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
+    """run_open_loop with the submitting and collecting threads in C++
+    (libpploadgen.so, through the same pp_server_* entry points of the library
+    instance that created `server`).  Same return dict."""
+    lg = _loadgen()
+    lib = server.lib
+    reqs = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    n = reqs.shape[0]
+    lat = np.zeros(n)
+    lag = np.zeros(n)
+    steps = np.zeros(n, dtype=np.uint64)
+    mk = ctypes.c_double()
+    fn = lambda f: ctypes.cast(f, ctypes.c_void_p)
+    dptr = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = lg.ppl_open_loop(server._h, fn(lib.pp_server_submit), fn(lib.pp_server_wait),
+                          fn(lib.pp_server_result), reqs.ctypes.data_as(ctypes.c_void_p), dptr(arr),
+                          n, int(hits_cap), float(timeout_s), dptr(lat), dptr(lag),
+                          steps.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(mk))
+    if rc == LOADGEN_TIMEOUT:
+        raise TimeoutError("open loop: requests not finished within %.0f s" % timeout_s)
+    if rc == LOADGEN_BAD_ARG:
+        raise ValueError("open loop: invalid arguments")
+    _abi.check(rc, lib)
+    return {
+        "latency_us": lat,
+        "submit_lag_us": lag,
+        "makespan_s": mk.value,
+        "photons": int(np.asarray(reqs["n_photons"], dtype=np.uint64).sum()),
+        "steps": int(steps.sum()),
+    }

@@ -147,6 +147,22 @@ def test_server_open_loop_poisson(ctx):
     assert out["steps"] > 9 * out["photons"]
 
 
+def test_server_open_loop_native_driver(ctx):
+    """The C++ load generator drives the same schedule through the same
+    entry points: every request finishes, and the photon steps (a property of
+    the requests alone) equal the Python driver's."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    arr, reqs = workloads.serving_requests(400, 5000.0, seed=6, photons=(1e3, 1e6))
+    with pp.Server(ctx, spec, max_slots=256, max_tiles_per_request=2048,
+                   hit_pool_records=1 << 20) as srv:
+        a = pp.run_open_loop_native(srv, arr, reqs, hits_cap=0)
+        b = pp.run_open_loop(srv, arr, reqs, hits_cap=0)
+    say("native open loop 5k req/s: p50 %.0f us p99 %.0f us, makespan %.3f s"
+        % (np.percentile(a["latency_us"], 50), np.percentile(a["latency_us"], 99), a["makespan_s"]))
+    assert np.all(a["latency_us"] > 0)
+    assert a["steps"] == b["steps"] > 9 * a["photons"]
+
+
 def test_server_queued_request_does_not_hold_the_kernel(ctx):
     """A request waiting for pool room does not keep the persistent kernel
     running: with A holding the whole pool and done, the kernel idles out

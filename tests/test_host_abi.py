@@ -30,6 +30,19 @@ def test_library_exports_every_declared_symbol():
     assert lib.pp_abi_version() == 1
 
 
+def test_loadgen_library_loads_and_rejects_bad_arguments():
+    """The serving load generator (libpploadgen.so) loads beside the library,
+    exports its driver, and refuses a NULL server without calling anything."""
+    from photonprop import serving
+
+    lg = serving._loadgen()
+    assert hasattr(lg, "ppl_open_loop")
+    mk = ctypes.c_double()
+    rc = lg.ppl_open_loop(None, None, None, None, None, None, 0, 0, 1.0, None, None, None,
+                          ctypes.byref(mk))
+    assert rc == serving.LOADGEN_BAD_ARG
+
+
 def test_struct_layouts_match_header():
     # sizes fixed by the header's comments / layout
     assert ctypes.sizeof(_abi.pp_summary) == 64
