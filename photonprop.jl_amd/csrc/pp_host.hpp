@@ -84,6 +84,10 @@ struct pp_ctx {
     pph::DevBuf scratch;
     // pinned staging for hit records going to the caller's host memory
     void* pinned[2] = {nullptr, nullptr};
+    // pinned buffer for a single request's upload and its summary download
+    // (pp_propagate), so that neither goes through the runtime's pageable path
+    void* io = nullptr;
+    bool occ_known = false;  // blocks_per_cu queried
     hipEvent_t copy_ev[2] = {nullptr, nullptr};
     std::vector<pph::DevReq> h_reqs;
     std::vector<uint64_t> h_tiles;

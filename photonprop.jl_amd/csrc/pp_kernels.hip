@@ -865,20 +865,35 @@ ppd::HG make_hg(float g) {
 }
 
 int grid_for(pp_ctx* c, uint64_t n_tiles, int* grid) {
-    int per_cu = 0;
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_kernel, kBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    c->blocks_per_cu = per_cu;
-    const uint64_t resident = (uint64_t)per_cu * (uint64_t)c->n_cu;
+    if (!c->occ_known) {
+        int per_cu = 0;
+        PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_kernel, kBlock, 0));
+        c->blocks_per_cu = per_cu < 1 ? 1 : per_cu;
+        c->occ_known = true;
+    }
+    const uint64_t resident = (uint64_t)c->blocks_per_cu * (uint64_t)c->n_cu;
     *grid = (int)std::min<uint64_t>(n_tiles, resident);
+    return PP_OK;
+}
+
+// Pinned I/O buffer of a context: one request's upload at 0, its summary and
+// hit cursor read back at kIoDownload.
+constexpr size_t kIoBytes = 4096;
+constexpr size_t kIoDownload = 2048;
+
+int ensure_io(pp_ctx* c) {
+    if (!c->io) PP_HIP(hipHostMalloc(&c->io, kIoBytes, 0));
     return PP_OK;
 }
 
 // Enqueue on c->stream: zero the counters, run the persistent propagation
 // kernel over all tiles of all requests, reduce per request into c->sums.
-// Hit records go to d_staging (capacity hits_cap).
+// Hit records go to d_staging (capacity hits_cap).  pinned_io: upload through
+// the context's pinned buffer (callers that synchronise before returning, so
+// that the buffer is free again at the next call).
 int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
-                        const pp_spectrum* spec, pp_hit* d_staging, uint64_t hits_cap) {
+                        const pp_spectrum* spec, pp_hit* d_staging, uint64_t hits_cap,
+                        bool pinned_io = false) {
     if (n_req == 0) return fail(PP_ERR_INVALID_ARG, "no requests");
     if (!reqs) return fail(PP_ERR_INVALID_ARG, "requests is NULL");
     SpecParams sp;
@@ -901,10 +916,21 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
     if ((rc = ensure(c->partials, std::max<uint64_t>(n_tiles, 1) * sizeof(TilePartial)))) return rc;
     if ((rc = ensure(c->counters, 64))) return rc;
     if ((rc = ensure(c->sums, n_req * sizeof(pp_summary)))) return rc;
-    PP_HIP(hipMemcpyAsync(c->reqs.p, c->h_reqs.data(), n_req * sizeof(DevReq),
-                          hipMemcpyHostToDevice, c->stream));
-    PP_HIP(hipMemcpyAsync(c->tile_start.p, c->h_tiles.data(), (n_req + 1) * sizeof(uint64_t),
-                          hipMemcpyHostToDevice, c->stream));
+    const size_t req_bytes = n_req * sizeof(DevReq), tile_bytes = (n_req + 1) * sizeof(uint64_t);
+    if (pinned_io && req_bytes + tile_bytes <= kIoDownload) {
+        if ((rc = ensure_io(c))) return rc;
+        char* io = static_cast<char*>(c->io);
+        memcpy(io, c->h_reqs.data(), req_bytes);
+        memcpy(io + req_bytes, c->h_tiles.data(), tile_bytes);
+        PP_HIP(hipMemcpyAsync(c->reqs.p, io, req_bytes, hipMemcpyHostToDevice, c->stream));
+        PP_HIP(hipMemcpyAsync(c->tile_start.p, io + req_bytes, tile_bytes, hipMemcpyHostToDevice,
+                              c->stream));
+    } else {
+        PP_HIP(hipMemcpyAsync(c->reqs.p, c->h_reqs.data(), req_bytes, hipMemcpyHostToDevice,
+                              c->stream));
+        PP_HIP(hipMemcpyAsync(c->tile_start.p, c->h_tiles.data(), tile_bytes,
+                              hipMemcpyHostToDevice, c->stream));
+    }
     // counters: [0] tile counter (u32), [2..3] hit cursor (u64)
     fill_u32_kernel<<<1, 64, 0, c->stream>>>((uint32_t*)c->counters.p, 16, 0u);
     PP_HIP(hipGetLastError());
@@ -1062,6 +1088,7 @@ int pp_destroy(pp_ctx* c) {
         if (c->pinned[h]) (void)hipHostFree(c->pinned[h]);
         if (c->copy_ev[h]) (void)hipEventDestroy(c->copy_ev[h]);
     }
+    if (c->io) (void)hipHostFree(c->io);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PP_OK;
@@ -1132,13 +1159,18 @@ int pp_propagate(pp_ctx* c, const pp_emitter* emitter, const pp_spectrum* spectr
     q.stream = stream;
     q.max_steps = max_steps;
     if ((rc = ensure(c->staging, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
-    if ((rc = enqueue_propagation(c, &q, 1, spectrum, (pp_hit*)c->staging.p, hits_cap))) return rc;
+    if ((rc = enqueue_propagation(c, &q, 1, spectrum, (pp_hit*)c->staging.p, hits_cap, true)))
+        return rc;
+    if ((rc = ensure_io(c))) return rc;
+    char* down = static_cast<char*>(c->io) + kIoDownload;
+    PP_HIP(hipMemcpyAsync(down, c->sums.p, sizeof(pp_summary), hipMemcpyDeviceToHost, c->stream));
+    PP_HIP(hipMemcpyAsync(down + sizeof(pp_summary), (uint32_t*)c->counters.p + 2,
+                          sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    PP_HIP(hipStreamSynchronize(c->stream));
     pp_summary s;
     unsigned long long cursor = 0;
-    PP_HIP(hipMemcpyAsync(&s, c->sums.p, sizeof s, hipMemcpyDeviceToHost, c->stream));
-    PP_HIP(hipMemcpyAsync(&cursor, (uint32_t*)c->counters.p + 2, sizeof cursor,
-                          hipMemcpyDeviceToHost, c->stream));
-    PP_HIP(hipStreamSynchronize(c->stream));
+    memcpy(&s, down, sizeof s);
+    memcpy(&cursor, down + sizeof(pp_summary), sizeof cursor);
     const uint64_t stored = std::min<uint64_t>(cursor, hits_cap);
     if (stored && (rc = copy_records_to_host(c, hits_out, c->staging.p, stored * sizeof(pp_hit))))
         return rc;
