@@ -180,4 +180,5 @@ def run_open_loop_native(server, arrivals, requests, hits_cap=0, timeout_s=120.0
         "makespan_s": mk.value,
         "photons": int(np.asarray(reqs["n_photons"], dtype=np.uint64).sum()),
         "steps": int(steps.sum()),
+        "steps_per_request": steps,
     }

@@ -163,6 +163,24 @@ def test_server_open_loop_native_driver(ctx):
     assert a["steps"] == b["steps"] > 9 * a["photons"]
 
 
+def test_server_entry_ring_wraps(ctx):
+    """40,000 one-tile requests publish more than twice the 16,384 range
+    entries of the ring (host copy and device copy alike): every request's
+    photon steps equal those of the same request in one batched launch, so
+    each tile found its own entry across the wraps."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    arr, reqs = workloads.serving_requests(40_000, 200_000.0, seed=8, photons=(500, 2000))
+    with pp.Server(ctx, spec, max_slots=1024, max_tiles_per_request=64,
+                   hit_pool_records=1 << 16) as srv:
+        out = pp.run_open_loop_native(srv, arr, reqs, hits_cap=0, timeout_s=60.0)
+        st = srv.stats()
+    _, _, sums = ctx.propagate_batch(reqs, spec, hits_cap=1, allow_overflow=True)
+    say("ring wrap: %d requests in %.3f s, %d tiles published"
+        % (len(reqs), out["makespan_s"], st["tiles_published"]))
+    assert st["completed"] == len(reqs)
+    np.testing.assert_array_equal(out["steps_per_request"], sums["n_steps"])
+
+
 def test_server_queued_request_does_not_hold_the_kernel(ctx):
     """A request waiting for pool room does not keep the persistent kernel
     running: with A holding the whole pool and done, the kernel idles out
