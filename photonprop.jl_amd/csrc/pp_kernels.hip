@@ -1357,8 +1357,13 @@ constexpr uint32_t kSrvRing = 16384;          // range-entry ring
 constexpr uint64_t kSrvExit = ~0ull;
 constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
 constexpr size_t kSrvGateOffset = 4096;  // ticket, gate and mirror counts on separate pages
-constexpr int kSrvVgprReserve = 64;      // VGPRs per SIMD lane left for other kernels
-constexpr int kSrvMaxWavesPerSimd = 4;   // of 8 wave slots per SIMD: 4 left for other kernels
+// Room left on every SIMD for other kernels: VGPRs per lane and wave slots (of
+// 8) the persistent grid does not take.  Without an SDMA engine the result
+// copies are runtime copy kernels that must find room beside it.
+constexpr int kSrvVgprReserve = 64;           // no SDMA engine
+constexpr int kSrvMaxWavesPerSimd = 4;
+constexpr int kSrvVgprReserveSdma = 32;       // result copies on an SDMA engine
+constexpr int kSrvMaxWavesPerSimdSdma = 5;
 constexpr uint64_t kSrvStageRecords = 1u << 16;  // 4 MiB per half of the result staging
 constexpr uint64_t kSrvChunkTiles = 64;     // smallest range entry of a large request
 constexpr uint64_t kSrvMaxChunks = 128;     // range entries per large request at most
@@ -1481,8 +1486,8 @@ __device__ void mirror_entries(const SrvParams& P, uint32_t lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
-// At most 96 VGPRs (5 waves per SIMD's worth): the grid takes 4 waves per
-// SIMD and (512 - kSrvVgprReserve) / VGPRs must allow that; 113 would allow 3.
+// At most 96 VGPRs (5 waves per SIMD's worth): the grid takes 4 or 5 waves per
+// SIMD and (512 - reserve) / VGPRs must allow that; 113 would allow 3 or 4.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) serve_kernel(SrvParams P) {
     __shared__ unsigned long long s_tile;
     __shared__ SrvSlot s_desc;
@@ -2098,17 +2103,21 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     // kernel waits (busy) for the request that the copy's pool release would
     // admit (7 waves per SIMD, leaving 8 or 64 VGPRs and one wave slot,
     // deadlocked this way).  Either way the grid leaves room on every CU for
-    // other kernels: each block puts one wave on each SIMD, and kSrvVgprReserve
-    // of the 512 VGPRs per lane and half the 8 wave slots stay free.  (A
-    // 64-VGPR build filling all 8 slots served the mixed request load 20-25 %
-    // slower than this one at 4: its spills cost more than the extra waves.)
+    // other kernels: each block puts one wave on each SIMD; without an SDMA
+    // engine 64 of the 512 VGPRs per lane and 4 of the 8 wave slots stay free,
+    // with one 32 VGPRs and 3 slots (5 waves per SIMD ran the same 64 x 1e8
+    // photons 7 % faster than 4: 113 against 121 ms).  (A 64-VGPR build
+    // filling all 8 slots served the mixed request load 20-25 % slower than one
+    // at 4: its spills cost more than the extra waves.)
     setup_sdma(s);
     int per_cu = 0;
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, 0));
     hipFuncAttributes fa;
     PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
     const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
-    per_cu = std::min({per_cu, (512 - kSrvVgprReserve) / vgpr, kSrvMaxWavesPerSimd});
+    const int reserve = s->sdma ? kSrvVgprReserveSdma : kSrvVgprReserve;
+    const int max_waves = s->sdma ? kSrvMaxWavesPerSimdSdma : kSrvMaxWavesPerSimd;
+    per_cu = std::min({per_cu, (512 - reserve) / vgpr, max_waves});
     s->grid = std::max(1, per_cu) * c->n_cu;
     s->free_slots.reserve(S);
     for (uint32_t i = 0; i < S; ++i) s->free_slots.push_back(S - 1 - i);
