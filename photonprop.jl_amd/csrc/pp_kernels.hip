@@ -8,17 +8,22 @@
 //   * One lane per photon.  A 256-lane block walks its tile in rounds of 256
 //     photons; every lane runs its photon to the hit or to max_steps.
 //   * Random numbers: Philox4x32-10 keyed by the request seed, counter
-//     (block, photon_lo, photon_hi, stream).  Each block yields six 21-bit
-//     uniforms; block 0 holds the emission draws {λ, φ, θ} and step 0, block
-//     b >= 1 holds steps 2b-1 and 2b.  Steps run in pairs that consume one
-//     block, so the bits a step reads are fixed at compile time.
-//   * Medium tables {n, λ_sca, λ_abs} and the spectrum knots sit in LDS.
-//   * Hits: wavefront ballot + mbcnt rank, one global atomic per wave that has
-//     any hit, 64-byte records written with 16-byte stores.
-//   * Sums: per-lane FP64 accumulators, fixed-order wave butterfly and block
-//     combine into one partial per tile; a second kernel adds a request's tile
-//     partials in tile order.  The grouping depends only on the request, so a
-//     request's sums are bitwise the same alone or inside any batch.
+//     (block, photon_lo, photon_hi, stream).  Each block yields four 23-bit
+//     uniforms and two 16-bit azimuths; block 0 holds the emission draws
+//     {λ, θ, φ} and step 0, block b >= 1 holds steps 2b-1 and 2b.  Steps run
+//     in pairs that consume one block, so the bits a step reads are fixed at
+//     compile time.
+//   * Medium tables {n, λ_sca, λ_abs} and the spectrum knots are read from
+//     device memory once per photon (they stay in L2).
+//   * Hits: each wave stages its records in LDS (ballot + mbcnt rank) and
+//     appends a batch of up to 64 with one global atomic and coalesced 16-byte
+//     stores.
+//   * Sums: steps in per-lane counters and a fixed-order butterfly; the hits'
+//     FP64 sums formed at each flush from the staged records by a fixed-order
+//     wave reduction; the 4 waves combined in order into one partial per tile;
+//     a second kernel adds a request's tile partials in tile order.  The
+//     grouping depends only on the request, so a request's sums are bitwise
+//     the same alone or inside any batch.
 #include <hip/hip_runtime.h>
 #include <hsa/hsa.h>
 #include <hsa/hsa_ext_amd.h>
