@@ -19,7 +19,8 @@ using ..Spectral
 using ..Emission
 using ..Detection
 
-export propagate_distance, propagate, PPHit, PPSummary, HIPContext
+export propagate_distance, propagate, propagate_batch, PPHit, PPSummary, HIPContext,
+       HIPServer, submit!, wait_done, take_result!
 
 const LIB = get(ENV, "PHOTONPROP_LIB", joinpath(@__DIR__, "..", "lib", "libphotonprop.so"))
 const C_VAC = 0.299792458  # m/ns
@@ -75,6 +76,22 @@ struct PPHit
     photon_id::UInt64
 end
 
+struct PPRequest
+    emitter::PPEmitter
+    target::PPSphere
+    seed::UInt64
+    stream::UInt32
+    max_steps::UInt32
+end
+
+struct PPServerConfig
+    max_slots::UInt32
+    max_tiles_per_request::UInt32
+    hit_pool_records::UInt64
+    idle_exit_us::UInt32
+    reserved::UInt32
+end
+
 struct PPSummary
     n_sim::UInt64
     n_steps::UInt64
@@ -95,6 +112,7 @@ const EMIT_CONE, EMIT_UNIFORM, EMIT_REF_ISOTROPIC = UInt32(0), UInt32(1), UInt32
 function __init__()
     @assert sizeof(PPHit) == 64 && sizeof(PPSummary) == 64 && sizeof(PPEmitter) == 48
     @assert sizeof(PPMediumTables) == 48 && sizeof(PPSpectrum) == 32
+    @assert sizeof(PPRequest) == 80 && sizeof(PPServerConfig) == 24
 end
 
 last_error() = unsafe_string(ccall((:pp_last_error, LIB), Cstring, ()))
@@ -149,6 +167,18 @@ profile_code(::AngularEmissionProfile{:CherenkovCone,T}) where {T} = EMIT_CONE
 profile_code(::AngularEmissionProfile{:UniformSphere,T}) where {T} = EMIT_UNIFORM
 profile_code(::AngularEmissionProfile{U,T}) where {U,T} = throw(ArgumentError("Cannot initialize $U"))
 
+emitter_struct(source::PhotonSource, beta::Real) =
+    PPEmitter(Tuple(Float32.(source.position)), Float32(source.time),
+              Tuple(Float32.(source.direction)), Float32(beta), UInt64(source.photons),
+              profile_code(source.emission_profile), 0)
+sphere_struct(target::DetectionSphere) = PPSphere(Tuple(Float32.(target.position)), Float32(target.radius))
+
+# (keep-alive, PPSpectrum) for a spectrum; the first element must outlive the call
+function spectrum_arg(s::Spectrum)
+    sk = spectrum_struct(s)
+    return sk isa Tuple{PPSpectrum,Nothing} ? (nothing, sk[1]) : sk
+end
+
 """
     propagate(ctx, source, target; seed=0, stream=0, max_steps=10, hits_cap=source.photons)
 
@@ -158,14 +188,11 @@ DetectionSphere.  Returns (Vector{PPHit}, PPSummary).
 function propagate(ctx::HIPContext, source::PhotonSource, target::DetectionSphere;
                    seed::Integer=0, stream::Integer=0, max_steps::Integer=10,
                    hits_cap::Integer=min(source.photons, 1 << 24), beta::Real=1.0)
-    em = PPEmitter(Tuple(Float32.(source.position)), Float32(source.time),
-                   Tuple(Float32.(source.direction)), Float32(beta), UInt64(source.photons),
-                   profile_code(source.emission_profile), 0)
-    sp = PPSphere(Tuple(Float32.(target.position)), Float32(target.radius))
+    em = emitter_struct(source, beta)
+    sp = sphere_struct(target)
     hits = Vector{PPHit}(undef, hits_cap)
     summ = Ref{PPSummary}()
-    sk = spectrum_struct(source.spectrum)
-    keep, spec = sk isa Tuple{PPSpectrum,Nothing} ? (nothing, sk[1]) : sk
+    keep, spec = spectrum_arg(source.spectrum)
     rc = GC.@preserve keep hits ccall((:pp_propagate, LIB), Cint,
         (Ptr{Cvoid}, Ref{PPEmitter}, Ref{PPSpectrum}, Ref{PPSphere}, UInt64, UInt32, UInt32,
          Ptr{PPHit}, UInt64, Ref{PPSummary}),
@@ -174,6 +201,112 @@ function propagate(ctx::HIPContext, source::PhotonSource, target::DetectionSpher
     rc == PP_ERR_OVERFLOW || check(rc)
     resize!(hits, Int(summ[].n_stored))
     return hits, summ[]
+end
+
+"""
+    propagate_batch(ctx, sources, targets; seeds=0:…, streams=0, max_steps=10,
+                    hits_cap=2^24, per_request_cap=0)
+
+Many (source, target) pairs in one persistent launch (pp_propagate_batch; the
+batching that src/photon_prop_cuda.jl:474-531 began).  All sources share the
+first source's spectrum.  Returns (hits, offsets, summaries): pair r owns
+hits[offsets[r]+1 : offsets[r+1]].
+"""
+function propagate_batch(ctx::HIPContext, sources::AbstractVector{<:PhotonSource},
+                         targets::AbstractVector{<:DetectionSphere};
+                         seeds=UInt64.(0:length(sources)-1), streams=zeros(UInt32, length(sources)),
+                         max_steps::Integer=10, hits_cap::Integer=1 << 24,
+                         per_request_cap::Integer=0, beta::Real=1.0)
+    n = length(sources)
+    @assert length(targets) == n
+    reqs = [PPRequest(emitter_struct(sources[i], beta), sphere_struct(targets[i]),
+                      UInt64(seeds[i]), UInt32(streams[i]), UInt32(max_steps)) for i in 1:n]
+    out_cap = per_request_cap > 0 ? min(hits_cap, per_request_cap * n) : hits_cap
+    hits = Vector{PPHit}(undef, out_cap)
+    offsets = zeros(UInt64, n + 1)
+    sums = Vector{PPSummary}(undef, n)
+    keep, spec = spectrum_arg(sources[1].spectrum)
+    rc = GC.@preserve keep reqs hits offsets sums ccall((:pp_propagate_batch, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{PPRequest}, UInt32, Ref{PPSpectrum}, Ptr{PPHit}, UInt64, UInt64,
+         Ptr{UInt64}, Ptr{PPSummary}),
+        ctx.handle, pointer(reqs), UInt32(n), spec, pointer(hits), UInt64(hits_cap),
+        UInt64(per_request_cap), pointer(offsets), pointer(sums))
+    rc == PP_ERR_OVERFLOW || check(rc)
+    resize!(hits, Int(offsets[end]))
+    return hits, offsets, sums
+end
+
+# ---- serving loop (pp_server_*) ---------------------------------------------
+
+"""
+    HIPServer(ctx, spectrum; max_slots=256, max_tiles_per_request=16384,
+              hit_pool_records=2^24, idle_exit_us=20000)
+
+Continuous batching of propagation requests into a persistent kernel.  Queue
+with `submit!`, poll with `wait_done`, collect with `take_result!`.
+"""
+mutable struct HIPServer
+    handle::Ptr{Cvoid}
+    ctx::HIPContext
+    keep::Any
+    function HIPServer(ctx::HIPContext, spectrum::Spectrum; max_slots::Integer=256,
+                       max_tiles_per_request::Integer=16384, hit_pool_records::Integer=1 << 24,
+                       idle_exit_us::Integer=20000)
+        cfg = PPServerConfig(UInt32(max_slots), UInt32(max_tiles_per_request),
+                             UInt64(hit_pool_records), UInt32(idle_exit_us), 0)
+        keep, spec = spectrum_arg(spectrum)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve keep check(ccall((:pp_server_create, LIB), Cint,
+            (Ptr{Cvoid}, Ref{PPSpectrum}, Ref{PPServerConfig}, Ptr{Ptr{Cvoid}}),
+            ctx.handle, spec, cfg, h))
+        srv = new(h[], ctx, keep)
+        finalizer(close, srv)
+        return srv
+    end
+end
+
+"""close(srv): drain and free the server now (before its context is freed)"""
+function Base.close(srv::HIPServer)
+    if srv.handle != C_NULL
+        ccall((:pp_server_destroy, LIB), Cint, (Ptr{Cvoid},), srv.handle)
+        srv.handle = C_NULL
+    end
+    return nothing
+end
+
+"""submit!(srv, source, target; seed, stream, max_steps=10, hits_cap=0) -> ticket (never blocks)"""
+function submit!(srv::HIPServer, source::PhotonSource, target::DetectionSphere;
+                 seed::Integer=0, stream::Integer=0, max_steps::Integer=10,
+                 hits_cap::Integer=0, beta::Real=1.0)
+    req = PPRequest(emitter_struct(source, beta), sphere_struct(target), UInt64(seed),
+                    UInt32(stream), UInt32(max_steps))
+    t = Ref{UInt64}(0)
+    check(ccall((:pp_server_submit, LIB), Cint, (Ptr{Cvoid}, Ref{PPRequest}, UInt64, Ref{UInt64}),
+                srv.handle, req, UInt64(hits_cap), t))
+    return t[]
+end
+
+"""wait_done(srv, ticket; timeout_ms=10000) -> true when the request has finished"""
+function wait_done(srv::HIPServer, ticket::Integer; timeout_ms::Integer=10000)
+    done = Ref{Cint}(0)
+    check(ccall((:pp_server_wait, LIB), Cint, (Ptr{Cvoid}, UInt64, UInt32, Ref{Cint}),
+                srv.handle, UInt64(ticket), UInt32(timeout_ms), done))
+    return done[] != 0
+end
+
+"""take_result!(srv, ticket; hits_cap=0) -> (hits, summary, latency_us); frees the ticket"""
+function take_result!(srv::HIPServer, ticket::Integer; hits_cap::Integer=0)
+    hits = Vector{PPHit}(undef, hits_cap)
+    summ = Ref{PPSummary}()
+    n = Ref{UInt64}(0)
+    lat = Ref{Float64}(0.0)
+    rc = GC.@preserve hits ccall((:pp_server_result, LIB), Cint,
+        (Ptr{Cvoid}, UInt64, Ref{PPSummary}, Ptr{PPHit}, UInt64, Ref{UInt64}, Ref{Float64}),
+        srv.handle, UInt64(ticket), summ, hits_cap > 0 ? pointer(hits) : C_NULL,
+        UInt64(hits_cap), n, lat)
+    rc == PP_ERR_OVERFLOW || check(rc)
+    resize!(hits, Int(n[]))
+    return hits, summ[], lat[]
 end
 
 const _ctx = Dict{UInt64,HIPContext}()
