@@ -30,6 +30,18 @@ def test_library_exports_every_declared_symbol():
     assert lib.pp_abi_version() == 1
 
 
+def test_julia_binding_calls_only_declared_symbols():
+    """photonprop.jl_amd/julia/PhotonPropHIP.jl (not executable here: no Julia)
+    ccalls only functions that the header declares and the library exports."""
+    jl = open(os.path.join(ROOT, "photonprop.jl_amd", "julia", "PhotonPropHIP.jl")).read()
+    used = set(re.findall(r"\(:(pp_[a-z0-9_]+), LIB\)", jl))
+    assert {"pp_create", "pp_propagate", "pp_propagate_batch", "pp_server_submit"} <= used
+    lib = _abi.load_library()
+    declared = set(declared_functions())
+    for n in sorted(used):
+        assert n in declared and hasattr(lib, n), n
+
+
 def test_loadgen_library_loads_and_rejects_bad_arguments():
     """The serving load generator (libpploadgen.so) loads beside the library,
     exports its driver, and refuses a NULL server without calling anything."""
