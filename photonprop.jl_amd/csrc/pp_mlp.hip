@@ -157,6 +157,39 @@ struct FusedArgs {
     const float* bias[kMaxLayers];
 };
 
+// One layer's k-loop for a width known at compile time (kSteps k-steps of 16),
+// fully unrolled: the weight fragments are prefetched kFusedAhead steps ahead
+// and the compiler waits for exactly the one each step needs (a rolled loop
+// waited for all of them at its back edge), and a step's four activation
+// fragments are read before its four MFMAs.  The loads are unconditional (a
+// lane past the layer's features reads row 0: its accumulator rows are output
+// features the epilogue does not keep), so the code is straight-line.  About
+// 2 % of the kernel's time; the epilogue below gave the rest of the gain.
+template <int kSteps>
+__device__ __forceinline__ void fused_mfma_steps(const bf16* wrow, const bf16* act, uint32_t li,
+                                                 uint32_t kh, f32x16 (&acc)[4]) {
+    bf16x8 wf[kFusedAhead];
+#pragma unroll
+    for (int u = 0; u < kFusedAhead && u < kSteps; ++u)
+        wf[u] = *reinterpret_cast<const bf16x8*>(wrow + u * 16);
+#pragma unroll
+    for (int st = 0; st < kSteps; ++st) {
+        const bf16x8 w = wf[st % kFusedAhead];
+        if (st + kFusedAhead < kSteps)
+            wf[st % kFusedAhead] = *reinterpret_cast<const bf16x8*>(wrow + (st + kFusedAhead) * 16);
+        // keep the prefetch here, kFusedAhead steps ahead of its use: left to
+        // itself the scheduler sinks each load next to its MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+        bf16x8 xb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            xb[t] = *reinterpret_cast<const bf16x8*>(act + (t * 32u + li) * kFusedLd + st * 16u + kh);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb[t], acc[t], 0, 0, 0);
+    }
+}
+
 __global__ void __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(4)))
 mlp_forward_fused_kernel(FusedArgs f) {
     extern __shared__ bf16 act[];  // [kFusedRows][kFusedLd]
@@ -185,7 +218,9 @@ mlp_forward_fused_kernel(FusedArgs f) {
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[t][q] = 0.0f;
-        if (mine) {
+        if (mine && K == kFusedMaxWidth) {
+            fused_mfma_steps<kFusedMaxWidth / 16>(wrow, act, li, kh, acc);
+        } else if (mine) {
             bf16x8 wf[kFusedAhead];
 #pragma unroll
             for (int u = 0; u < kFusedAhead; ++u)
@@ -211,6 +246,17 @@ mlp_forward_fused_kernel(FusedArgs f) {
         __syncthreads();  // every wave has read this layer's input
         if (mine) {
             const float* bias = f.bias[l];
+            // this lane's 16 biases (D rows 8g + 4*(lane/32) + r), loaded once
+            // per layer rather than once per row tile, and SiLU computed for
+            // every lane and then selected: the guarded per-element loads and
+            // branches this replaces took a fifth of the kernel's time
+            float bj[4][4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const uint32_t jg = wave * 32u + 8u * g + 4u * (lane >> 5);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) bj[g][r] = jg + r < n_true ? bias[jg + r] : 0.0f;
+            }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const uint32_t i = t * 32u + li;
@@ -220,14 +266,16 @@ mlp_forward_fused_kernel(FusedArgs f) {
                     const uint32_t jg = wave * 32u + 8u * g + 4u * (lane >> 5);
                     float v[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        v[r] = acc[t][4 * g + r] + (jg + r < n_true ? bias[jg + r] : 0.0f);
+                    for (int r = 0; r < 4; ++r) v[r] = acc[t][4 * g + r] + bj[g][r];
                     if (!last) {
                         typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
                         bf16x4 y;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            y[r] = (bf16)(jg + r < n_true ? silu(v[r]) : 0.0f);
+                        for (int r = 0; r < 4; ++r) {
+                            // computed for every lane, then selected: no branch
+                            const float sv = silu(v[r]);
+                            y[r] = (bf16)(jg + r < n_true ? sv : 0.0f);
+                        }
                         *reinterpret_cast<bf16x4*>(act + i * kFusedLd + jg) = y;
                     } else if (row0 + i < f.M) {
 #pragma unroll
