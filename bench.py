@@ -211,7 +211,7 @@ def bench_serving(pp, ctx, spectrum, n_req):
 
 
 def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, n_eval=512, eval_photons=20_000_000):
-    """Train the BF16-MFMA surrogate on MC rows from the batched kernel, then judge
+    """Train the BF16-MFMA surrogate on MC rows from the batched kernel, then evaluate
     it against high-statistics MC on held-out pairs: median relative yield error
     (pairs with >= 400 weighted hits, i.e. <= 5% MC noise) and the median
     Kolmogorov distance between the predicted gamma-mixture CDF and the MC
@@ -349,8 +349,8 @@ def main():
         mlp.close()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU; --gpus %d is not supported (the path shards by photon-id "
-              "range across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU; --gpus %d is not supported (the path would shard by "
+              "photon-id range, one range per GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch
