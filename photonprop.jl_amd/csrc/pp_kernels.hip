@@ -1115,6 +1115,22 @@ namespace {
 // copy of the one before.  hipMemcpy straight into pageable memory stages
 // through the runtime's own small buffers, at a fraction of the link's rate.
 constexpr uint64_t kHostStageBytes = 16ull << 20;
+constexpr int kHostCopyThreads = 4;
+
+// memcpy split over kHostCopyThreads threads: the first touch of fresh
+// pageable memory (a page fault per 4 KiB) bounds one thread well below the
+// DMA rate.
+void parallel_memcpy(char* d, const char* s, uint64_t n) {
+    const uint64_t part = ((n + kHostCopyThreads - 1) / kHostCopyThreads + 4095) & ~4095ull;
+    std::thread th[kHostCopyThreads - 1];
+    for (int i = 1; i < kHostCopyThreads; ++i) {
+        const uint64_t o = (uint64_t)i * part;
+        if (o < n) th[i - 1] = std::thread([=] { memcpy(d + o, s + o, std::min(part, n - o)); });
+    }
+    memcpy(d, s, std::min(part, n));
+    for (auto& t : th)
+        if (t.joinable()) t.join();
+}
 
 int copy_records_to_host(pp_ctx* c, void* dst, const void* src, uint64_t bytes) {
     if (bytes <= kHostStageBytes / 4) {
@@ -1139,7 +1155,8 @@ int copy_records_to_host(pp_ctx* c, void* dst, const void* src, uint64_t bytes) 
             PP_HIP(hipEventRecord(c->copy_ev[h ^ 1], c->stream));
         }
         PP_HIP(hipEventSynchronize(c->copy_ev[h]));
-        memcpy(d + off, c->pinned[h], std::min(kHostStageBytes, bytes - off));
+        parallel_memcpy(d + off, static_cast<const char*>(c->pinned[h]),
+                        std::min(kHostStageBytes, bytes - off));
     }
     return PP_OK;
 }
