@@ -1119,8 +1119,12 @@ constexpr int kHostCopyThreads = 4;
 
 // memcpy split over kHostCopyThreads threads: the first touch of fresh
 // pageable memory (a page fault per 4 KiB) bounds one thread well below the
-// DMA rate.
+// DMA rate.  Below 1 MiB the thread starts would cost more than they save.
 void parallel_memcpy(char* d, const char* s, uint64_t n) {
+    if (n < (1u << 20)) {
+        memcpy(d, s, n);
+        return;
+    }
     const uint64_t part = ((n + kHostCopyThreads - 1) / kHostCopyThreads + 4095) & ~4095ull;
     std::thread th[kHostCopyThreads - 1];
     for (int i = 1; i < kHostCopyThreads; ++i) {
@@ -2261,7 +2265,9 @@ int pp_server_result(pp_server* s, uint64_t ticket, pp_summary* summary, pp_hit*
             if (next < n && !sdma_issue(s, h ^ 1, src + next, std::min(n - next, kSrvStageRecords)))
                 return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed to start",
                             (unsigned long long)ticket);
-            memcpy(hits_out + off, s->stage + (uint64_t)h * kSrvStageRecords, m * sizeof(pp_hit));
+            parallel_memcpy(reinterpret_cast<char*>(hits_out + off),
+                            reinterpret_cast<const char*>(s->stage + (uint64_t)h * kSrvStageRecords),
+                            m * sizeof(pp_hit));
             off = next;
             h ^= 1;
         }
