@@ -150,8 +150,12 @@ typedef struct pp_request {
     pp_sphere target;
     uint64_t seed;          /* Philox key */
     uint32_t stream;        /* Philox counter word 3 */
-    uint32_t max_steps;     /* scattering-loop cap; reference uses 10 (src/photon_prop_cuda.jl:319) */
+    uint32_t max_steps;     /* scattering-loop cap, 1 .. PP_MAX_STEPS; reference uses 10
+                               (src/photon_prop_cuda.jl:319) */
 } pp_request;
+
+/* Largest max_steps any call accepts (2^24 scattering-loop iterations per photon). */
+#define PP_MAX_STEPS (1u << 24)
 
 typedef struct pp_ctx pp_ctx;
 

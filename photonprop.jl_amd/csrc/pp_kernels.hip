@@ -124,7 +124,7 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -165,7 +165,7 @@ __device__ __forceinline__ void store_quad(uint4* dst, const uint4& v) {
 template <bool kSysStores>
 __device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit* hits,
                                            uint64_t hits_cap, unsigned long long* cursor,
-                                           uint32_t lane, double* wsum, uint32_t* whits) {
+                                           uint32_t lane, double* wsum, uint64_t* whits) {
     double w = 0.0, wt = 0.0, wt2 = 0.0;
     if (lane < n) {
         const double t = (double)__uint_as_float(buf[4 * lane].w);
@@ -213,7 +213,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                                                 uint32_t req_index, uint64_t first,
                                                 uint32_t count, pp_hit* hits, uint64_t hits_cap,
                                                 unsigned long long* hit_cursor,
-                                                double (*s_red)[3], uint32_t (*s_redi)[2],
+                                                double (*s_red)[3], uint64_t (*s_redi)[2],
                                                 uint4 (*s_hitbuf)[kHitBuf * 4]) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -227,13 +227,15 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         s_red[wave][0] = 0.0;
         s_red[wave][1] = 0.0;
         s_red[wave][2] = 0.0;
-        s_redi[wave][1] = 0u;
+        s_redi[wave][1] = 0ull;
     }
     const uint32_t rounds = (count + kBlock - 1) / kBlock;
     const uint32_t max_steps = R.max_steps;
     const float r2 = R.r2;
     const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
 
+    // steps of this lane's photons: at most kRoundsPerTile * max_steps <= 2^29
+    // (max_steps <= 2^24); the wave's sum is taken in 64 bits
     uint32_t acc_steps = 0;
 
     for (uint32_t k = 0; k < rounds; ++k) {
@@ -394,7 +396,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                                &s_redi[wave][1]);
 
     // ---- tile partial: fixed-order reduction ----
-    const uint32_t ws = wave_sum_u32(acc_steps);
+    const unsigned long long ws = wave_sum_u64(acc_steps);
     if (lane == 0) s_redi[wave][0] = ws;
     __syncthreads();
     TilePartial tp;
@@ -421,7 +423,7 @@ __device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevR
                                                     uint32_t count, pp_hit* hits,
                                                     uint64_t hits_cap,
                                                     unsigned long long* hit_cursor,
-                                                    double (*s_red)[3], uint32_t (*s_redi)[2],
+                                                    double (*s_red)[3], uint64_t (*s_redi)[2],
                                                     uint4 (*s_hitbuf)[kHitBuf * 4]) {
     if (T.hg.iso)
         return run_tile<kSysStores, true>(T, R, req_index, first, count, hits, hits_cap,
@@ -448,7 +450,7 @@ __device__ __forceinline__ TileCtx make_tile_ctx(const float4* table, uint32_t n
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) propagate_kernel(KParams P) {
     __shared__ uint32_t s_tile;
     __shared__ double s_red[kWaves][3];
-    __shared__ uint32_t s_redi[kWaves][2];
+    __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     for (;;) {
@@ -824,7 +826,7 @@ int make_devreq(const pp_request* q, uint32_t idx, DevReq* d) {
     if (e.profile == PP_EMIT_CHERENKOV_CONE && !(e.beta > 0.0f && std::isfinite(e.beta)))
         return fail(PP_ERR_INVALID_ARG, "request %u: beta %g must be > 0", idx, (double)e.beta);
     if (q->max_steps == 0) return fail(PP_ERR_INVALID_ARG, "request %u: max_steps must be >= 1", idx);
-    if (q->max_steps > (1u << 28))
+    if (q->max_steps > PP_MAX_STEPS)
         return fail(PP_ERR_INVALID_ARG, "request %u: max_steps %u too large", idx, q->max_steps);
     memset(d, 0, sizeof *d);
     for (int j = 0; j < 3; ++j) {
@@ -1521,7 +1523,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     __shared__ uint32_t s_slot;
     __shared__ uint32_t s_last;
     __shared__ double s_red[kWaves][3];
-    __shared__ uint32_t s_redi[kWaves][2];
+    __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
@@ -1699,6 +1701,7 @@ struct pp_server {
     int grid = 0;
     uint32_t n_points = 0;
     float lmin = 0.0f, inv_dl = 0.0f, g = 0.0f;
+    float n_min = 0.0f;          // smallest n of the table snapshot (Cherenkov check)
     hipStream_t kstream = nullptr, cstream = nullptr;
     hipEvent_t kdone = nullptr;
     bool kernel_live = false;
@@ -1883,10 +1886,27 @@ void publish_locked(pp_server* s) {
 }
 
 // Admit pending jobs (FIFO) while a slot and pool room exist, then publish.
+// A job without photons completes at once and takes neither a slot nor pool
+// records (its hits_cap reservation is zero), so no slot ever carries the
+// ticket of a job that the device will not complete.
 void admit_locked(pp_server* s) {
     if (s->stopping) return;  // tickets restart at resume_tile on the next launch
-    while (!s->pending.empty() && !s->free_slots.empty()) {
-        Job& j = s->jobs[s->pending.front()];
+    while (!s->pending.empty()) {
+        auto it = s->jobs.find(s->pending.front());
+        if (it == s->jobs.end()) {  // cannot happen: pending jobs are never erased
+            s->pending.pop_front();
+            continue;
+        }
+        Job& j = it->second;
+        if (j.n_tiles == 0) {
+            s->pending.pop_front();
+            memset(&j.summary, 0, sizeof j.summary);
+            j.done = true;
+            j.t_done = std::chrono::steady_clock::now();
+            ++s->completed;
+            continue;
+        }
+        if (s->free_slots.empty()) break;
         uint64_t off = 0;
         if (!pool_alloc(s, j.hits_cap, &off)) break;
         s->pending.pop_front();
@@ -1895,14 +1915,6 @@ void admit_locked(pp_server* s) {
         j.hit_base = off;
         j.seq = s->next_seq++;
         s->slot_ticket[j.slot] = j.ticket;
-        if (j.n_tiles == 0) {  // nothing to run: complete at once
-            memset(&j.summary, 0, sizeof j.summary);
-            j.done = true;
-            j.t_done = std::chrono::steady_clock::now();
-            s->free_slots.push_back(j.slot);
-            ++s->completed;
-            continue;
-        }
         SrvSlot sd;
         memset(&sd, 0, sizeof sd);
         make_devreq(&j.req, j.slot, &sd.req);
@@ -1923,7 +1935,11 @@ bool reap_locked(pp_server* s) {
     for (uint32_t slot = 0; slot < s->cfg.max_slots; ++slot) {
         const uint64_t tk = s->slot_ticket[slot];
         if (!tk) continue;
-        Job& j = s->jobs[tk];
+        // a slot's ticket is cleared when its job completes, before the job
+        // can be collected and erased; never insert a job here
+        auto it = s->jobs.find(tk);
+        if (it == s->jobs.end()) continue;
+        Job& j = it->second;
         if (j.done) continue;
         if (__atomic_load_n(&s->done[slot].seq, __ATOMIC_ACQUIRE) != j.seq) continue;
         memcpy(&j.summary, s->done[slot].words, sizeof(pp_summary));
@@ -2096,6 +2112,7 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     s->lmin = c->lmin;
     s->inv_dl = c->inv_dl;
     s->g = c->g;
+    s->n_min = c->n_min;
     if (s->spec.kind == PP_SPEC_KNOTS)
         PP_HIP(hipMemcpyAsync(s->knots.p, c->knots.p, s->spec.n_knots * sizeof(float),
                               hipMemcpyDeviceToDevice, c->stream));
@@ -2192,7 +2209,8 @@ int pp_server_submit(pp_server* s, const pp_request* request, uint64_t hits_cap,
     DevReq dr;
     int rc = make_devreq(request, 0, &dr);
     if (rc) return rc;
-    if ((rc = check_cone(s->ctx, request, 1, s->ctx->n_min))) return rc;
+    // against the server's own table snapshot, not the context's current medium
+    if ((rc = check_cone(s->ctx, request, 1, s->n_min))) return rc;
     const uint64_t n_tiles = (request->emitter.n_photons + kTilePhotons - 1) / kTilePhotons;
     if (n_tiles > s->cfg.max_tiles_per_request)
         return fail(PP_ERR_INVALID_ARG, "request of %llu photons exceeds %u tiles of %d",
@@ -2206,7 +2224,7 @@ int pp_server_submit(pp_server* s, const pp_request* request, uint64_t hits_cap,
     Job j;
     j.ticket = s->next_ticket++;
     j.req = *request;
-    j.hits_cap = hits_cap;
+    j.hits_cap = n_tiles ? hits_cap : 0;  // a job without photons reserves no records
     j.n_tiles = n_tiles;
     j.t_submit = std::chrono::steady_clock::now();
     s->jobs[j.ticket] = j;

@@ -30,6 +30,7 @@ PP_EMIT_REF_ISOTROPIC = 2
 
 PP_MAX_TABLE_POINTS = 2048
 PP_MAX_KNOTS = 1024
+PP_MAX_STEPS = 1 << 24
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 

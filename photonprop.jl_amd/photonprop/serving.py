@@ -79,15 +79,20 @@ class Server:
 
 def run_open_loop(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
     """Submit requests[i] at arrivals[i] seconds (Poisson schedule) from one
-    thread while another collects results in order.  Returns a dict with
-    per-request latencies in µs from the scheduled arrival to completion (the
-    library's submit-to-completion time plus any lag of the submitting thread
-    behind the schedule), the makespan and the photon / step totals."""
+    thread while another collects results in order.  hits_cap is one capacity
+    for every request or one per request.  Returns a dict with per-request
+    latencies in µs from the scheduled arrival to completion (the library's
+    submit-to-completion time plus any lag of the submitting thread behind the
+    schedule), the makespan, the photon / step totals, every request's summary
+    and the hits of the requests with a nonzero capacity (by index)."""
     n = len(requests)
+    caps = np.broadcast_to(np.asarray(hits_cap, dtype=np.uint64), (n,))
     tickets = [None] * n
     lag = np.zeros(n)
     lat = np.zeros(n)
     steps = np.zeros(n, dtype=np.uint64)
+    summaries = [None] * n
+    kept = {}
     submitted = threading.Semaphore(0)
     err = []
 
@@ -98,9 +103,12 @@ def run_open_loop(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
                 while not server.wait(tickets[i], 1000):
                     if time.perf_counter() - t0 > timeout_s:
                         raise TimeoutError("request %d not finished" % i)
-                s, _, l = server.result(tickets[i], hits_cap)
+                s, h, l = server.result(tickets[i], int(caps[i]))
                 lat[i] = l
                 steps[i] = s["n_steps"]
+                summaries[i] = s
+                if caps[i]:
+                    kept[i] = h
         except Exception as e:  # surfaced after join
             err.append(e)
 
@@ -112,7 +120,7 @@ def run_open_loop(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
         if delay > 0:
             time.sleep(delay)
         lag[i] = 1e6 * max(0.0, time.perf_counter() - t0 - arrivals[i])
-        tickets[i] = server.submit(requests[i], hits_cap)
+        tickets[i] = server.submit(requests[i], int(caps[i]))
         submitted.release()
     th.join()
     makespan = time.perf_counter() - t0
@@ -124,6 +132,8 @@ def run_open_loop(server, arrivals, requests, hits_cap=0, timeout_s=120.0):
         "makespan_s": makespan,
         "photons": int(np.asarray(requests["n_photons"], dtype=np.uint64).sum()),
         "steps": int(steps.sum()),
+        "summaries": summaries,
+        "hits": kept,
     }
 
 

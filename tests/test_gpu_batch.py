@@ -128,3 +128,33 @@ def test_per_request_cap_keeps_a_subset(ctx):
         full = full_h[int(full_o[r]):int(full_o[r + 1])]
         assert seg.shape[0] == min(3, full.shape[0])
         assert set(seg["photon_id"].tolist()) <= set(full["photon_id"].tolist())
+
+
+def test_cascade_light_sources_batched_equal_alone():
+    """The light-yield generator's requests (a 1 TeV electron cascade as point
+    sources every 0.5 m along its axis, src/lightyield.jl:215-285) in the
+    reference's cascadia water with its 20-knot Cherenkov spectrum, through
+    one batched launch: every source's summary is bitwise, and its hit set by
+    photon id equal to, the same source propagated alone, and each hit arrives
+    no earlier than its source's delayed start t0."""
+    from photonprop import lightyield as ly
+
+    m = pp.make_cascadia_medium_properties()
+    spec = pp.CherenkovSpectrum((300.0, 800.0), 20, m)
+    p = ly.Particle((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 5.0, 1e3, ly.ParticleType.EMinus)
+    tgt = pp.DetectionSphere((3.0, 0.0, 6.0), 0.3)
+    reqs = ly.particle_to_elongated_lightsource(p, (0.0, 20.0), 0.5, m, (300.0, 800.0), tgt, seed=4)
+    with pp.Context(m) as c:
+        hits, off, sums = c.propagate_batch(reqs, spec)
+        print("cascade: %d sources, %.3g photons, %d hits" % (len(reqs), float(reqs["n_photons"].sum()),
+                                                             hits.shape[0]), flush=True)
+        np.testing.assert_array_equal(sums["n_sim"], reqs["n_photons"])
+        assert np.all(sums["status"] == 0) and int(sums["n_hits"].sum()) > 1000
+        for r in range(len(reqs)):
+            seg = hits[int(off[r]):int(off[r + 1])]
+            assert np.all(seg["time"] >= reqs["t0"][r])
+        for r in range(0, len(reqs), 5):
+            res, _, _ = alone(c, reqs[r], spec)
+            for k in SUM_KEYS:
+                assert res.summary[k] == sums[k][r], (r, k, res.summary[k], sums[k][r])
+            np.testing.assert_array_equal(strip(res.hits), strip(hits[int(off[r]):int(off[r + 1])]))

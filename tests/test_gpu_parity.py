@@ -17,6 +17,8 @@ import oracle
 import photonprop as pp
 from oracle.compare import assert_parity, compare_hits, oracle_for
 
+from closed_forms import C_LIGHT, beta_below, check_unscattered_hits
+
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -170,7 +172,10 @@ def test_per_photon_parity(ctx, case):
     flips = cmp["only_gpu"] + cmp["only_cpu"]
     assert abs(int(s["n_steps"]) - int(rsum["n_steps"])) <= max_steps * flips
     if rsum["n_hits"]:
-        assert s["sum_w"] == pytest.approx(rsum["sum_w"], rel=1e-4 + 2.0 * flips / rsum["n_hits"])
+        # the FP64 weight and time moments, each to the same flip-based tolerance
+        tol = 1e-4 + 2.0 * flips / rsum["n_hits"]
+        for k in ("sum_w", "sum_wt", "sum_wt2"):
+            assert s[k] == pytest.approx(rsum[k], rel=tol), (k, s[k], rsum[k])
     # every hit lies on the sphere surface and has a unit direction
     if res.hits.shape[0]:
         r = np.linalg.norm(res.hits["pos"].astype(np.float64) - np.array(centre), axis=1)
@@ -286,6 +291,94 @@ def test_device_survival_exp_minus_d_over_abs():
     np.testing.assert_allclose(res.hits["dist"], d - r, rtol=1e-6)
 
 
+def test_device_straight_line_hit_time_and_time_moments():
+    """Photons emitted straight along the axis into a sphere on the axis: every
+    hit's time is t0 + (d - r)·n/c (phase index, src/photon_prop_cuda.jl:634-638)
+    and the FP64 moments are t·Σw and t²·Σw."""
+    n, d, r, la, t0 = 1.34, 12.0, 0.21, 25.0, 17.25
+    with pp.Context(ConstMedium(n=n, abs_len=la)) as c:
+        src = pp.PhotonSource((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), t0, 1_000_000,
+                              pp.Monochromatic(450.0), pp.AngularEmissionProfile("CherenkovCone"),
+                              beta_below(n))
+        res = c.propagate(src, pp.DetectionSphere((0.0, 0.0, d), r))
+    s = res.summary
+    L = d - r
+    t = t0 + L * n / C_LIGHT
+    say("straight line: time", float(res.hits["time"][0]), "want", t, s)
+    assert s["n_hits"] == src.photons == res.hits.shape[0]
+    np.testing.assert_allclose(res.hits["dist"], L, rtol=1e-6)
+    np.testing.assert_allclose(res.hits["time"], t, rtol=1e-6)
+    np.testing.assert_allclose(res.hits["weight"], math.exp(-L / la), rtol=1e-6)
+    assert s["sum_w"] == pytest.approx(src.photons * math.exp(-L / la), rel=1e-6)
+    assert s["sum_wt"] == pytest.approx(s["sum_w"] * t, rel=1e-6)
+    assert s["sum_wt2"] == pytest.approx(s["sum_w"] * t * t, rel=1e-6)
+
+
+def test_device_unscattered_hit_time_weight_and_moments(ctx):
+    """Uniform emission in the configured seawater, one step allowed: every
+    hit is an unscattered photon, whose time t0 + L·n(λ)/c, weight
+    exp(-L/λ_abs(λ)) and path length follow from the geometry and the medium's
+    formulas; the FP64 moments are the sums over the records."""
+    centre, r, t0 = (1.1, -0.4, 2.3), 0.5, 3.75
+    src = pp.PhotonSource((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), t0, 4_000_000,
+                          pp.InverseSquareSpectrum(300.0, 600.0),
+                          pp.AngularEmissionProfile("UniformSphere"))
+    res = ctx.propagate(src, pp.DetectionSphere(centre, r), seed=12, max_steps=1)
+    say("unscattered closed form:", res.summary)
+    assert res.summary["n_hits"] > 20_000
+    check_unscattered_hits(res.hits, res.summary, (0.0, 0.0, 0.0), centre, r, t0)
+
+
+class RisingIndexMedium(ConstMedium):
+    """n rising linearly from 1.33 at 300 nm to 1.35 at 800 nm (n_min at 300 nm)."""
+
+    def __init__(self):
+        super().__init__(n=1.33, abs_len=30.0)
+        self.table_range = (300.0, 800.0)
+        self.table_points = 501
+
+    def refractive_index(self, w):
+        return 1.33 + 0.02 * (np.asarray(w, dtype=float) - 300.0) / 500.0
+
+
+def test_result_table_tres_straight_line():
+    """result_table's tres (src/photon_prop_cuda.jl:639-641) for photons flying
+    straight to a sphere on the axis at 300 nm: L = d - r, so
+    tres = (d - r)·n(300)/c - (d - r)·n(800)/c, with the geometric time taken at
+    n(800 nm); initial θ = 0 and ref_ix = n(300)."""
+    m = RisingIndexMedium()
+    d, r = 20.0, 0.21
+    with pp.Context(m) as c:
+        src = pp.PhotonSource((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0.0, 20_000, pp.Monochromatic(300.0),
+                              pp.AngularEmissionProfile("CherenkovCone"), beta_below(1.33))
+        res = c.propagate(src, pp.DetectionSphere((0.0, 0.0, d), r))
+    assert res.summary["n_hits"] == src.photons
+    df = pp.propagation.result_table(res.hits, m, d, r)
+    want = (d - r) * 1.33 / C_LIGHT - (d - r) * 1.35 / C_LIGHT
+    say("tres", float(df["tres"].iloc[0]), "want", want)
+    # FP32 path length: 1e-7 of the ~88 ns photon time, i.e. ~1e-5 ns
+    np.testing.assert_allclose(df["tres"], want, rtol=0, atol=5e-5)
+    np.testing.assert_allclose(df["ref_ix"], 1.33, rtol=1e-12)
+    np.testing.assert_allclose(df["initial_theta"], 0.0, atol=1e-3)
+    np.testing.assert_allclose(df["dist_travelled"], d - r, rtol=1e-6)
+
+
+def test_device_step_count_at_large_max_steps():
+    """A target no photon can reach: every photon runs all max_steps = 2^20
+    iterations, so n_steps = N·max_steps exactly (the per-tile step sums are
+    64-bit above the lane level); max_steps above PP_MAX_STEPS is refused."""
+    N, max_steps = 300, 1 << 20
+    with pp.Context(ConstMedium(n=1.34, abs_len=20.0, sca_len=1.0)) as c:
+        src = cone_source(N, spectrum=pp.Monochromatic(450.0))
+        res = c.propagate(src, pp.DetectionSphere((1.0e6, 0.0, 0.0), 0.21), max_steps=max_steps)
+        say("large max_steps:", res.summary)
+        assert res.summary["n_hits"] == 0
+        assert res.summary["n_steps"] == N * max_steps
+        with pytest.raises(pp.PhotonPropError, match="max_steps"):
+            c.propagate(src, pp.DetectionSphere((1.0, 0.0, 0.0), 0.21),
+                        max_steps=pp._abi.PP_MAX_STEPS + 1)
+
+
 def test_device_no_scatter_cone_fraction():
     n, d, r = 1.34, 5.0, 0.4
     thc = math.acos(1 / n)
@@ -360,14 +453,20 @@ def test_statistical_equivalence_independent_seeds(ctx, dist, angle):
         p = stats.ks_2samp(np.asarray(fg, np.float64), np.asarray(fc, np.float64)).pvalue
         say("KS", name, p)
         assert p > 0.01, (name, p)
-    # χ² on the arrival-time histogram (weighted by counts, equal-probability bins)
-    edges = np.quantile(ch["time"], np.linspace(0, 1, 11))
-    edges[0], edges[-1] = -np.inf, np.inf
-    cg = np.histogram(g.hits["time"], edges)[0]
-    cc = np.histogram(ch["time"], edges)[0]
-    chi2, p, _, _ = stats.chi2_contingency(np.vstack([cg, cc]))
-    say("chi2 time", chi2, p)
-    assert p > 0.01
+    # χ² homogeneity of the arrival-time, wavelength and impact-angle histograms
+    # (counts in 10 bins of equal probability under the oracle's sample)
+    for name, fg, fc in (("time", g.hits["time"], ch["time"]),
+                         ("wavelength", g.hits["wavelength"], ch["wavelength"]),
+                         ("impact", impact(g.hits), impact(ch))):
+        fg = np.asarray(fg, np.float64)
+        fc = np.asarray(fc, np.float64)
+        edges = np.unique(np.quantile(fc, np.linspace(0, 1, 11)))
+        edges[0], edges[-1] = -np.inf, np.inf
+        cg = np.histogram(fg, edges)[0]
+        cc = np.histogram(fc, edges)[0]
+        chi2, p, _, _ = stats.chi2_contingency(np.vstack([cg, cc]))
+        say("chi2", name, chi2, p)
+        assert p > 0.01, (name, p)
 
 
 # ------------------------------------------------------------- failure paths ----
@@ -423,9 +522,11 @@ def test_scale_geometries_statistics_vs_fp64_oracle(ctx):
     geometry) against the FP64 oracle at 1e7 (seed 2).  For every geometry with
     >= 200 oracle hits: the yield Σw/N within 4 sigma (the oracle's error from its
     hit sample, the GPU's from its hit count and the oracle's per-hit E[w²]) and
-    a two-sample KS test of the arrival times with p > 1e-4; over all of them the
+    two-sample KS tests with p > 1e-4; over all of them the
     yields' Σz² follows χ²(k) and Fisher's combination of the KS p-values
-    χ²(2k), each with p > 0.001.  (The arrival times are heavy-tailed, so their
+    χ²(2k), each with p > 0.001.  The KS tests cover the arrival time, the
+    wavelength and the impact angle (cosine between the arrival direction and
+    the inward surface normal).  (The arrival times are heavy-tailed, so their
     mean is compared through the whole distribution rather than a z-score.)"""
     from photonprop import workloads
 
@@ -435,7 +536,7 @@ def test_scale_geometries_statistics_vs_fp64_oracle(ctx):
     gh, off, gs = ctx.propagate_batch(reqs, spec, per_request_cap=keep, hits_cap=len(reqs) * keep * 40,
                                       allow_overflow=True)
     assert np.all(gs["n_sim"] == n_gpu)
-    zs, ps = [], []
+    zs, ps = [], {"time": [], "wavelength": [], "impact": []}
     for i, r in enumerate(reqs):
         src = cone_source(n_cpu)
         tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
@@ -449,15 +550,28 @@ def test_scale_geometries_statistics_vs_fp64_oracle(ctx):
         vc = ((w ** 2).sum() / n_cpu - yc ** 2) / n_cpu
         vg = ((w ** 2).mean() * ng / n_gpu - yg ** 2) / n_gpu
         zy = (yg - yc) / math.sqrt(vc + vg)
-        tg = gh["time"][int(off[i]):int(off[i + 1])].astype(np.float64)
-        p = stats.ks_2samp(tg, ch["time"].astype(np.float64)).pvalue
-        say("geom %2d d=%6.2f hits gpu %8d (kept %5d) cpu %7d  yield z=%+.2f  KS(time) p=%.3f"
-            % (i, np.linalg.norm(r["center"]), ng, tg.size, cs["n_hits"], zy, p))
-        assert abs(zy) < 4 and p > 1e-4, (i, zy, p)
+        seg = gh[int(off[i]):int(off[i + 1])]
+        centre = np.asarray(r["center"], np.float64)
+
+        def impact(h):
+            nrm = (h["pos"].astype(np.float64) - centre) / float(r["radius"])
+            return np.einsum("ij,ij->i", -h["dir"].astype(np.float64), nrm)
+
+        pv = {}
+        for name, fg, fc in (("time", seg["time"], ch["time"]),
+                             ("wavelength", seg["wavelength"], ch["wavelength"]),
+                             ("impact", impact(seg), impact(ch))):
+            pv[name] = stats.ks_2samp(np.asarray(fg, np.float64), np.asarray(fc, np.float64)).pvalue
+        say("geom %2d d=%6.2f hits gpu %8d (kept %5d) cpu %7d  yield z=%+.2f  KS p time %.3f "
+            "wavelength %.3f impact %.3f" % (i, np.linalg.norm(r["center"]), ng, seg.shape[0],
+                                            cs["n_hits"], zy, pv["time"], pv["wavelength"],
+                                            pv["impact"]))
+        assert abs(zy) < 4 and min(pv.values()) > 1e-4, (i, zy, pv)
         zs.append(zy)
-        ps.append(p)
+        for name in ps:
+            ps[name].append(pv[name])
     k = len(zs)
     p_y = stats.chi2.sf(float(np.sum(np.square(zs))), k)
-    p_t = stats.chi2.sf(float(-2.0 * np.sum(np.log(ps))), 2 * k)
-    say("geometries compared %d: yield chi2 p %.3f, Fisher KS p %.3f" % (k, p_y, p_t))
-    assert k >= 20 and p_y > 0.001 and p_t > 0.001
+    p_f = {name: stats.chi2.sf(float(-2.0 * np.sum(np.log(v))), 2 * k) for name, v in ps.items()}
+    say("geometries compared %d: yield chi2 p %.3f, Fisher KS p %s" % (k, p_y, p_f))
+    assert k >= 20 and p_y > 0.001 and min(p_f.values()) > 0.001

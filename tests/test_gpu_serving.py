@@ -237,3 +237,99 @@ def test_server_idle_stop_and_relaunch(ctx):
         for k in SUM_KEYS:
             assert s[k] == ref.summary[k], (k, s[k], ref.summary[k])
         np.testing.assert_array_equal(key(h), key(ref.hits))
+
+
+def test_server_zero_photon_request_takes_no_slot(ctx):
+    """A request without photons completes at once and takes neither a slot
+    nor pool records.  On a fresh server, collecting it and letting the worker
+    poll leaves completed == submitted and nothing in flight (the slot it would
+    have held is not reaped as a phantom completion); the ticket is then
+    unknown, and two requests submitted next each equal Context.propagate
+    alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(3, seed=71)
+    reqs["n_photons"] = [0, 150_000, 90_000]
+    pool = 1 << 18
+    with pp.Server(ctx, spec, max_slots=2, max_tiles_per_request=64, hit_pool_records=pool) as srv:
+        t0 = srv.submit(reqs[0], 1000)
+        assert srv.wait(t0, 10000)
+        s0, h0, _ = srv.result(t0, 1000)
+        assert s0["n_sim"] == 0 and s0["n_hits"] == 0 and h0.shape[0] == 0
+        time.sleep(0.03)  # several worker passes
+        st = srv.stats()
+        say("after the zero-photon request:", st)
+        assert st["submitted"] == st["completed"] == 1
+        assert st["in_flight"] == 0 and st["pending"] == 0 and st["pool_records_free"] == pool
+        with pytest.raises(pp.PhotonPropError) as e:
+            srv.wait(t0, 10)
+        assert e.value.code == pp._abi.PP_ERR_NOT_FOUND
+        ta = srv.submit(reqs[1], 150_000)
+        tb = srv.submit(reqs[2], 90_000)
+        assert srv.wait(ta, 60000) and srv.wait(tb, 60000)
+        got = [srv.result(ta, 150_000), srv.result(tb, 90_000)]
+        st = srv.stats()
+    assert st["submitted"] == st["completed"] == 3
+    for r, (s, h, _) in zip(reqs[1:], got):
+        ref = alone(ctx, r)
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (k, s[k], ref.summary[k])
+        np.testing.assert_array_equal(key(h), key(ref.hits))
+
+
+@pytest.mark.parametrize("rate", [2000.0, 5000.0])
+def test_server_serving_workload_full_size(ctx, rate):
+    """The serving workload at its stated size: 400 requests, Poisson arrivals
+    at `rate`/s, budgets log-uniform on 1e4..1e8 photons, random emitter pose,
+    targets (r = 0.21 m) at log-uniform 1..150 m.  Every request completes
+    (p50/p99 reported).  Nine requests keep their hits -- the four largest
+    budgets beyond 20 m (all above 1e7 photons), four across the budget range
+    and the nearest target among budgets up to 3e6 photons: their summaries
+    are bitwise, and their hit sets by photon id equal, those of pp_propagate
+    alone.  The last of them is also checked photon by photon against the
+    oracle."""
+    from oracle.compare import assert_parity, compare_hits, oracle_for
+
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    arr, reqs = workloads.serving_requests(400, rate, seed=17, photons=(1e4, 1e8),
+                                           distance=(1.0, 150.0))
+    n = len(reqs)
+    dist = np.linalg.norm(reqs["center"], axis=1)
+    big = [i for i in np.argsort(reqs["n_photons"])[::-1] if dist[i] > 20.0][:4]
+    order = np.argsort(reqs["n_photons"])
+    spread = [int(order[j]) for j in (0, 120, 240, 360)]
+    small = np.nonzero(reqs["n_photons"] <= 3_000_000)[0]
+    near = int(small[np.argmin(dist[small])])  # the oracle's request: many hits, few photons
+    pick = sorted(set(int(i) for i in big) | set(spread) | {near})
+    assert len(pick) == 9 and min(int(reqs["n_photons"][i]) for i in big) > 10_000_000
+    caps = np.zeros(n, dtype=np.uint64)
+    caps[pick] = 1 << 20
+    say("serving workload at %d/s: %d requests, %.3g photons; kept hits of %s (photons %s)"
+        % (rate, n, float(reqs["n_photons"].sum()), pick, [int(reqs["n_photons"][i]) for i in pick]))
+    with pp.Server(ctx, spec, max_slots=1024, max_tiles_per_request=16384,
+                   hit_pool_records=1 << 24) as srv:
+        out = pp.run_open_loop(srv, arr, reqs, hits_cap=caps, timeout_s=100.0)
+        st = srv.stats()
+    lat = out["latency_us"]
+    say("serving workload %d/s: p50 %.2f ms p99 %.2f ms, %.3g photons/s, launches %d"
+        % (rate, np.percentile(lat, 50) / 1e3, np.percentile(lat, 99) / 1e3,
+           out["photons"] / out["makespan_s"], st["kernel_launches"]))
+    assert st["submitted"] == st["completed"] == n
+    assert all(s is not None and s["n_sim"] == r["n_photons"] for s, r in zip(out["summaries"], reqs))
+    for i in pick:
+        r = reqs[i]
+        s, h = out["summaries"][i], out["hits"][i]
+        src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), float(r["t0"]), int(r["n_photons"]),
+                              spec, pp.AngularEmissionProfile("CherenkovCone"), float(r["beta"]))
+        tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
+        ref = ctx.propagate(src, tgt, seed=int(r["seed"]), stream=int(r["stream"]),
+                            max_steps=int(r["max_steps"]), hits_cap=1 << 20)
+        assert s["n_hits"] <= (1 << 20), "cap too small for an exact hit comparison"
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (i, k, s[k], ref.summary[k])
+        np.testing.assert_array_equal(key(h), key(ref.hits))
+        if i == near:
+            orc, osum = oracle_for(ctx, src, tgt, seed=int(r["seed"]), stream=int(r["stream"]))
+            cmp = compare_hits(h, orc)
+            say("request %d against the oracle:" % i, cmp)
+            assert osum["n_hits"] > 20
+            assert_parity(cmp, osum["n_hits"])

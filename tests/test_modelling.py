@@ -104,7 +104,30 @@ def test_fit_photon_dist_on_a_synthetic_table():
 
 @pytest.mark.gpu
 def test_make_photon_fits_end_to_end():
-    fits = md.make_photon_fits(2_000_000, 3, 4, max_dist=30.0)
+    """make_photon_fits on the GPU (src/modelling.jl:86-111); one cell's
+    det_fraction is recomputed by hand from the raw hit records of the same
+    propagation (the reference's setup: cascadia water, 20-knot spectrum,
+    :IsotropicEmission, sphere r = 0.21 m on the z axis, seed 0):
+    Σ f(cos(θ0 - obs))/(2∫f) · w · PMT(λ) / N with θ0 the initial polar angle."""
+    n_ph = 2_000_000
+    fits = md.make_photon_fits(n_ph, 3, 4, max_dist=30.0)
     assert set(fits.columns) == {"obs_angle", "fit_alpha", "fit_theta", "det_fraction", "distance"}
     assert len(fits) > 0 and np.all(np.isfinite(fits["fit_alpha"]))
     assert np.all(fits["det_fraction"] >= 0)
+    row = fits.iloc[len(fits) - 1]
+    m = pp.make_cascadia_medium_properties()
+    src = pp.PhotonSource((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0.0, n_ph,
+                          pp.CherenkovSpectrum((300.0, 800.0), 20, m),
+                          pp.AngularEmissionProfile("IsotropicEmission"))
+    with pp.Context(m) as c:
+        h = c.propagate(src, pp.DetectionSphere((0.0, 0.0, float(row["distance"])), 0.21), seed=0).hits
+    theta0 = np.arccos(np.clip(h["init_dir"][:, 2].astype(np.float64), -1.0, 1.0))
+    lam = h["wavelength"].astype(np.float64)
+    n = pp.medium.get_refractive_index_fry(lam, m.salinity, m.temperature, m.pressure)
+    f = cherenkov_ang_dist(np.cos(theta0 - float(row["obs_angle"])), n)
+    norm = 2.0 * md.cherenkov_ang_dist_integral(n)
+    hand = float((f / norm * h["weight"].astype(np.float64) * p_one_pmt_acc(lam)).sum() / n_ph)
+    print("det_fraction", float(row["det_fraction"]), "by hand", hand, flush=True)
+    assert hand > 0
+    # the table interpolates the angular integral over n in steps of 0.01
+    assert float(row["det_fraction"]) == pytest.approx(hand, rel=1e-4)

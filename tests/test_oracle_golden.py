@@ -14,6 +14,8 @@ import photonprop as pp
 from photonprop import _abi
 from photonprop.propagation import emitter_struct, spectrum_struct, sphere_struct
 
+from closed_forms import C_LIGHT, beta_below, check_unscattered_hits
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -239,7 +241,7 @@ def test_ray_sphere_brute_force():
 
 # ---------------------------------------------------------- emission ----
 
-def _structs(medium, spectrum, profile, axis=(0.0, 0.0, 1.0), n=1000, beta=1.0):
+def _structs(medium, spectrum, profile, axis=(0.0, 0.0, 1.0), n=1000, beta=1.0, t0=0.0):
     ctx_tables = pp.medium_tables(medium)
     keep = [np.ascontiguousarray(a, dtype=np.float32)
             for a in (ctx_tables.n_phase, ctx_tables.abs_len, ctx_tables.sca_len)]
@@ -247,7 +249,7 @@ def _structs(medium, spectrum, profile, axis=(0.0, 0.0, 1.0), n=1000, beta=1.0):
     t.lambda_min, t.lambda_max, t.n_points = ctx_tables.lambda_min, ctx_tables.lambda_max, ctx_tables.n_points
     t.n_phase, t.abs_len, t.sca_len = [a.ctypes.data_as(_abi.c_float_p) for a in keep]
     t.g = ctx_tables.g
-    src = pp.PhotonSource((0, 0, 0), axis, 0.0, n, spectrum, pp.AngularEmissionProfile(profile), beta)
+    src = pp.PhotonSource((0, 0, 0), axis, t0, n, spectrum, pp.AngularEmissionProfile(profile), beta)
     sp, k2 = spectrum_struct(spectrum)
     return t, sp, emitter_struct(src), (keep, k2, ctx_tables)
 
@@ -316,7 +318,7 @@ class _Med(pp.medium.MediumProperties):
 def _run_oracle(medium, source, target, max_steps=10, seed=0, precision=32, n_threads=4):
     from photonprop._abi import HIT_DTYPE, pp_summary
     t, sp, em, keep = _structs(medium, source.spectrum, source.emission_profile.kind,
-                               source.direction, source.photons, source.beta)
+                               source.direction, source.photons, source.beta, source.time)
     hits, summ = oracle.propagate(t, sp, em, sphere_struct(target), seed=seed, max_steps=max_steps,
                                   precision=precision, n_threads=n_threads, hit_dtype=HIT_DTYPE,
                                   summary_struct=pp_summary)
@@ -388,3 +390,43 @@ def test_fp32_and_fp64_oracles_agree_per_photon():
     assert s32.n_hits > 500
     assert len(common) >= s64.n_hits - max(3, 1e-3 * s64.n_hits)
     assert abs(s32.sum_w - s64.sum_w) < 1e-3 * s64.sum_w
+
+
+# --------------------------------------------- hit time and FP64 moments ----
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_straight_line_hit_time_and_time_moments(precision):
+    """Photons emitted straight along the axis into a sphere on the axis
+    (constant n, no scattering): every hit has L = d - r, time
+    t0 + (d - r)·n/c (the phase index, src/photon_prop_cuda.jl:634-638) and
+    weight exp(-(d - r)/λ_abs), so Σw·t = t·Σw and Σw·t² = t²·Σw with that one t."""
+    n, d, r, la, t0 = 1.34, 12.0, 0.21, 25.0, 17.25
+    med = _Med(n=n, abs_len=la)
+    src = pp.PhotonSource((0, 0, 0), (0, 0, 1), t0, 5_000, pp.Monochromatic(450.0),
+                          pp.AngularEmissionProfile("CherenkovCone"), beta=beta_below(n))
+    hits, summ = _run_oracle(med, src, pp.DetectionSphere((0.0, 0.0, d), r), precision=precision)
+    L = d - r
+    t = t0 + L * n / C_LIGHT
+    w = math.exp(-L / la)
+    assert summ.n_hits == src.photons == hits.shape[0]
+    np.testing.assert_allclose(hits["dist"], L, rtol=1e-6)
+    np.testing.assert_allclose(hits["time"], t, rtol=1e-6)
+    np.testing.assert_allclose(hits["weight"], w, rtol=1e-6)
+    assert summ.sum_w == pytest.approx(src.photons * w, rel=1e-6)
+    assert summ.sum_wt == pytest.approx(summ.sum_w * t, rel=1e-6)
+    assert summ.sum_wt2 == pytest.approx(summ.sum_w * t * t, rel=1e-6)
+
+
+def test_unscattered_hit_time_weight_and_moments_closed_form():
+    """Uniform emission in the configured seawater with one step allowed: the
+    hits are the unscattered photons, whose time, weight and path length follow
+    from the geometry and the medium's formulas alone (a wavelength-dependent
+    n, so a misplaced or mis-indexed n fails)."""
+    m = pp.SeawaterConfigMedium()
+    centre, r, t0 = (1.1, -0.4, 2.3), 0.5, 3.75
+    src = pp.PhotonSource((0, 0, 0), (0, 0, 1), t0, 200_000, pp.InverseSquareSpectrum(300, 600),
+                          pp.AngularEmissionProfile("UniformSphere"))
+    hits, summ = _run_oracle(m, src, pp.DetectionSphere(centre, r), max_steps=1)
+    s = {k: getattr(summ, k) for k in ("n_stored", "n_hits", "sum_w", "sum_wt", "sum_wt2")}
+    assert s["n_hits"] > 1000
+    check_unscattered_hits(hits, s, (0, 0, 0), centre, r, t0)
