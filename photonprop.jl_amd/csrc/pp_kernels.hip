@@ -769,8 +769,8 @@ int upload_medium(pp_ctx* c, const pp_medium_tables* m) {
     return PP_OK;
 }
 
-// Validate a spectrum and fill the kernel parameters; knots go to the device.
-int prepare_spectrum(pp_ctx* c, const pp_spectrum* s, SpecParams* out) {
+// Validate a spectrum and fill the kernel parameters (host only).
+int check_spectrum(const pp_spectrum* s, SpecParams* out) {
     if (!s) return fail(PP_ERR_INVALID_ARG, "spectrum is NULL");
     memset(out, 0, sizeof *out);
     out->kind = s->kind;
@@ -794,10 +794,6 @@ int prepare_spectrum(pp_ctx* c, const pp_spectrum* s, SpecParams* out) {
         for (uint32_t i = 0; i < s->n_knots; ++i)
             if (!(s->knots[i] > 0.0f) || !std::isfinite(s->knots[i]))
                 return fail(PP_ERR_INVALID_ARG, "spectrum knot %u = %g invalid", i, (double)s->knots[i]);
-        int rc = ensure(c->knots, PP_MAX_KNOTS * sizeof(float));
-        if (rc) return rc;
-        PP_HIP(hipMemcpyAsync(c->knots.p, s->knots, s->n_knots * sizeof(float),
-                              hipMemcpyHostToDevice, c->stream));
         out->n_knots = s->n_knots;
         out->knot_step = 1.0f / (float)(s->n_knots - 1);
         return PP_OK;
@@ -805,6 +801,15 @@ int prepare_spectrum(pp_ctx* c, const pp_spectrum* s, SpecParams* out) {
     default:
         return fail(PP_ERR_UNSUPPORTED, "Cannot initialize spectrum kind %u", s->kind);
     }
+}
+
+// A checked spectrum's knots to d_knots (PP_MAX_KNOTS floats) on `stream`;
+// nothing for the other kinds.
+int upload_knots(const pp_spectrum* s, float* d_knots, hipStream_t stream) {
+    if (s->kind != PP_SPEC_KNOTS) return PP_OK;
+    PP_HIP(hipMemcpyAsync(d_knots, s->knots, s->n_knots * sizeof(float), hipMemcpyHostToDevice,
+                          stream));
+    return PP_OK;
 }
 
 int make_devreq(const pp_request* q, uint32_t idx, DevReq* d) {
@@ -893,18 +898,29 @@ int ensure_io(pp_ctx* c) {
     return PP_OK;
 }
 
-// Enqueue on c->stream: zero the counters, run the persistent propagation
-// kernel over all tiles of all requests, reduce per request into c->sums.
-// Hit records go to d_staging (capacity hits_cap).  pinned_io: upload through
-// the context's pinned buffer (callers that synchronise before returning, so
-// that the buffer is free again at the next call).
-int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
-                        const pp_spectrum* spec, pp_hit* d_staging, uint64_t hits_cap,
-                        bool pinned_io = false) {
+// Device work buffers of one launch.  The allocating entry points take them
+// from the context's grow-only buffers (ctx_work), the allocation-free one
+// from the fixed per-device scratch array (scratch_work).
+struct WorkBufs {
+    DevReq* reqs;           // [n_req]
+    uint64_t* tile_start;   // [n_req + 1]
+    TilePartial* partials;  // [n_tiles]
+    uint32_t* counters;     // [16]: [0] tile counter, [2..3] hit cursor
+    pp_summary* sums;       // [n_req]
+    float* knots;           // [PP_MAX_KNOTS] (knot spectra)
+    pp_hit* staging;        // [hits_cap] hit records as the kernel appends them
+    uint32_t* cnt;          // [n_req] compaction: records per request
+    uint32_t* fill;         // [n_req] compaction: records placed per request
+    uint32_t* rec_req;      // [hits_cap] compaction: request of each staged record
+};
+
+// Host side of a launch: check the spectrum and the requests, and build the
+// kernel's request records and tile offsets in c->h_reqs / c->h_tiles.
+int plan_launch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spectrum* spec,
+                SpecParams* sp, uint64_t* n_tiles_out) {
     if (n_req == 0) return fail(PP_ERR_INVALID_ARG, "no requests");
     if (!reqs) return fail(PP_ERR_INVALID_ARG, "requests is NULL");
-    SpecParams sp;
-    int rc = prepare_spectrum(c, spec, &sp);
+    int rc = check_spectrum(spec, sp);
     if (rc) return rc;
     if ((rc = check_cone(c, reqs, n_req, c->n_min))) return rc;
     c->h_reqs.resize(n_req);
@@ -918,50 +934,88 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
     if (n_tiles > 0xFFFFFFFFull - (1ull << 24))
         return fail(PP_ERR_INVALID_ARG, "too many photons in one launch (%llu tiles)",
                     (unsigned long long)n_tiles);
+    *n_tiles_out = n_tiles;
+    return PP_OK;
+}
+
+// The context's work buffers, grown to fit (compaction: also its buffers).
+int ctx_work(pp_ctx* c, uint32_t n_req, uint64_t n_tiles, uint64_t hits_cap, bool knots,
+             bool compaction, WorkBufs* W) {
+    int rc;
     if ((rc = ensure(c->reqs, n_req * sizeof(DevReq)))) return rc;
     if ((rc = ensure(c->tile_start, (n_req + 1) * sizeof(uint64_t)))) return rc;
     if ((rc = ensure(c->partials, std::max<uint64_t>(n_tiles, 1) * sizeof(TilePartial)))) return rc;
     if ((rc = ensure(c->counters, 64))) return rc;
     if ((rc = ensure(c->sums, n_req * sizeof(pp_summary)))) return rc;
+    if (knots && (rc = ensure(c->knots, PP_MAX_KNOTS * sizeof(float)))) return rc;
+    if ((rc = ensure(c->staging, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
+    if (compaction) {
+        if ((rc = ensure(c->cnt, n_req * sizeof(uint32_t)))) return rc;
+        if ((rc = ensure(c->fill, n_req * sizeof(uint32_t)))) return rc;
+        if ((rc = ensure(c->rec_req, std::max<uint64_t>(hits_cap, 1) * sizeof(uint32_t)))) return rc;
+    }
+    W->reqs = (DevReq*)c->reqs.p;
+    W->tile_start = (uint64_t*)c->tile_start.p;
+    W->partials = (TilePartial*)c->partials.p;
+    W->counters = (uint32_t*)c->counters.p;
+    W->sums = (pp_summary*)c->sums.p;
+    W->knots = (float*)c->knots.p;
+    W->staging = (pp_hit*)c->staging.p;
+    W->cnt = (uint32_t*)c->cnt.p;
+    W->fill = (uint32_t*)c->fill.p;
+    W->rec_req = (uint32_t*)c->rec_req.p;
+    return PP_OK;
+}
+
+// Enqueue on c->stream: knots and requests up, zero the counters, run the
+// persistent propagation kernel over all tiles of all requests (hit records to
+// W.staging, capacity hits_cap), reduce per request into W.sums.  pinned_io:
+// upload through the context's pinned buffer (callers that synchronise before
+// returning, so that the buffer is free again at the next call).
+int enqueue_propagation(pp_ctx* c, uint32_t n_req, uint64_t n_tiles, const SpecParams& sp,
+                        const pp_spectrum* spec, const WorkBufs& W, uint64_t hits_cap,
+                        bool pinned_io = false) {
+    int rc;
+    if ((rc = upload_knots(spec, W.knots, c->stream))) return rc;
     const size_t req_bytes = n_req * sizeof(DevReq), tile_bytes = (n_req + 1) * sizeof(uint64_t);
     if (pinned_io && req_bytes + tile_bytes <= kIoDownload) {
         if ((rc = ensure_io(c))) return rc;
         char* io = static_cast<char*>(c->io);
         memcpy(io, c->h_reqs.data(), req_bytes);
         memcpy(io + req_bytes, c->h_tiles.data(), tile_bytes);
-        PP_HIP(hipMemcpyAsync(c->reqs.p, io, req_bytes, hipMemcpyHostToDevice, c->stream));
-        PP_HIP(hipMemcpyAsync(c->tile_start.p, io + req_bytes, tile_bytes, hipMemcpyHostToDevice,
+        PP_HIP(hipMemcpyAsync(W.reqs, io, req_bytes, hipMemcpyHostToDevice, c->stream));
+        PP_HIP(hipMemcpyAsync(W.tile_start, io + req_bytes, tile_bytes, hipMemcpyHostToDevice,
                               c->stream));
     } else {
-        PP_HIP(hipMemcpyAsync(c->reqs.p, c->h_reqs.data(), req_bytes, hipMemcpyHostToDevice,
+        PP_HIP(hipMemcpyAsync(W.reqs, c->h_reqs.data(), req_bytes, hipMemcpyHostToDevice,
                               c->stream));
-        PP_HIP(hipMemcpyAsync(c->tile_start.p, c->h_tiles.data(), tile_bytes,
-                              hipMemcpyHostToDevice, c->stream));
+        PP_HIP(hipMemcpyAsync(W.tile_start, c->h_tiles.data(), tile_bytes, hipMemcpyHostToDevice,
+                              c->stream));
     }
     // counters: [0] tile counter (u32), [2..3] hit cursor (u64)
-    fill_u32_kernel<<<1, 64, 0, c->stream>>>((uint32_t*)c->counters.p, 16, 0u);
+    fill_u32_kernel<<<1, 64, 0, c->stream>>>(W.counters, 16, 0u);
     PP_HIP(hipGetLastError());
 
     int grid = 0;
     if ((rc = grid_for(c, n_tiles, &grid))) return rc;
     KParams P;
     memset(&P, 0, sizeof P);
-    P.reqs = (const DevReq*)c->reqs.p;
-    P.tile_start = (const uint64_t*)c->tile_start.p;
+    P.reqs = W.reqs;
+    P.tile_start = W.tile_start;
     P.n_req = n_req;
     P.n_points = c->n_points;
     P.n_tiles = n_tiles;
-    P.tile_counter = (uint32_t*)c->counters.p;
+    P.tile_counter = W.counters;
     P.table = (const float4*)c->table.p;
-    P.knots = (const float*)c->knots.p;
+    P.knots = W.knots;
     P.lmin = c->lmin;
     P.inv_dl = c->inv_dl;
     P.hg = make_hg(c->g);
     P.spec = sp;
-    P.hits = d_staging;
+    P.hits = W.staging;
     P.hits_cap = hits_cap;
-    P.hit_cursor = (unsigned long long*)((uint32_t*)c->counters.p + 2);
-    P.partials = (TilePartial*)c->partials.p;
+    P.hit_cursor = (unsigned long long*)(W.counters + 2);
+    P.partials = W.partials;
     PP_HIP(hipEventRecord(c->ev0, c->stream));
     if (grid > 0) {
         propagate_kernel<<<grid, kBlock, 0, c->stream>>>(P);
@@ -969,33 +1023,24 @@ int enqueue_propagation(pp_ctx* c, const pp_request* reqs, uint32_t n_req,
     }
     PP_HIP(hipEventRecord(c->ev1, c->stream));
     c->timed = true;
-    reduce_kernel<<<n_req, 64, 0, c->stream>>>((const TilePartial*)c->partials.p,
-                                               (const uint64_t*)c->tile_start.p,
-                                               (const DevReq*)c->reqs.p, n_req,
-                                               (pp_summary*)c->sums.p);
+    reduce_kernel<<<n_req, 64, 0, c->stream>>>(W.partials, W.tile_start, W.reqs, n_req, W.sums);
     PP_HIP(hipGetLastError());
     return PP_OK;
 }
 
-// Group staged hits by request into d_out with offsets; fills n_stored/status.
-int enqueue_compaction(pp_ctx* c, uint32_t n_req, const pp_hit* d_staging, uint64_t hits_cap,
+// Group the staged hits by request into d_out with offsets; fills n_stored and
+// status of W.sums.
+int enqueue_compaction(pp_ctx* c, uint32_t n_req, const WorkBufs& W, uint64_t hits_cap,
                        uint64_t per_cap, pp_hit* d_out, uint64_t* d_offsets) {
-    int rc;
-    if ((rc = ensure(c->cnt, n_req * sizeof(uint32_t)))) return rc;
-    if ((rc = ensure(c->fill, n_req * sizeof(uint32_t)))) return rc;
-    if ((rc = ensure(c->rec_req, std::max<uint64_t>(hits_cap, 1) * sizeof(uint32_t)))) return rc;
     const int zb = (int)std::min<uint64_t>(1024, (n_req + 255) / 256);
-    fill_u32_kernel<<<zb, 256, 0, c->stream>>>((uint32_t*)c->cnt.p, n_req, 0u);
-    fill_u32_kernel<<<zb, 256, 0, c->stream>>>((uint32_t*)c->fill.p, n_req, 0u);
-    const unsigned long long* cursor = (const unsigned long long*)((uint32_t*)c->counters.p + 2);
+    fill_u32_kernel<<<zb, 256, 0, c->stream>>>(W.cnt, n_req, 0u);
+    fill_u32_kernel<<<zb, 256, 0, c->stream>>>(W.fill, n_req, 0u);
+    const unsigned long long* cursor = (const unsigned long long*)(W.counters + 2);
     const int gb = 2048;
-    count_kernel<<<gb, 256, 0, c->stream>>>(d_staging, cursor, hits_cap, n_req,
-                                            (uint32_t*)c->cnt.p, (uint32_t*)c->rec_req.p);
-    scan_kernel<<<1, 1024, 0, c->stream>>>((const uint32_t*)c->cnt.p, n_req, per_cap, d_offsets,
-                                           (pp_summary*)c->sums.p);
-    scatter_kernel<<<gb, 256, 0, c->stream>>>(d_staging, (const uint32_t*)c->rec_req.p, cursor,
-                                              hits_cap, n_req, per_cap, d_offsets,
-                                              (uint32_t*)c->fill.p, d_out);
+    count_kernel<<<gb, 256, 0, c->stream>>>(W.staging, cursor, hits_cap, n_req, W.cnt, W.rec_req);
+    scan_kernel<<<1, 1024, 0, c->stream>>>(W.cnt, n_req, per_cap, d_offsets, W.sums);
+    scatter_kernel<<<gb, 256, 0, c->stream>>>(W.staging, W.rec_req, cursor, hits_cap, n_req, per_cap,
+                                              d_offsets, W.fill, d_out);
     PP_HIP(hipGetLastError());
     return PP_OK;
 }
@@ -1013,14 +1058,21 @@ int check_records(uint64_t n, const char* what) {
     return PP_OK;
 }
 
+// Propagation and compaction of a batch into d_out / d_offsets, summaries in
+// the context's buffer (returned in *sums).
 int run_batch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spectrum* spec,
-              pp_hit* d_out, uint64_t hits_cap, uint64_t per_cap, uint64_t* d_offsets) {
+              pp_hit* d_out, uint64_t hits_cap, uint64_t per_cap, uint64_t* d_offsets,
+              pp_summary** sums) {
     int rc;
     if ((rc = check_records(hits_cap, "hits_cap"))) return rc;
-    if ((rc = ensure(c->staging, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
-    if ((rc = enqueue_propagation(c, reqs, n_req, spec, (pp_hit*)c->staging.p, hits_cap))) return rc;
-    return enqueue_compaction(c, n_req, (const pp_hit*)c->staging.p, hits_cap, per_cap, d_out,
-                              d_offsets);
+    SpecParams sp;
+    uint64_t n_tiles = 0;
+    if ((rc = plan_launch(c, reqs, n_req, spec, &sp, &n_tiles))) return rc;
+    WorkBufs W;
+    if ((rc = ctx_work(c, n_req, n_tiles, hits_cap, sp.kind == PP_SPEC_KNOTS, true, &W))) return rc;
+    if ((rc = enqueue_propagation(c, n_req, n_tiles, sp, spec, W, hits_cap))) return rc;
+    *sums = W.sums;
+    return enqueue_compaction(c, n_req, W, hits_cap, per_cap, d_out, d_offsets);
 }
 
 // Records the compacted output of a batch can hold.
@@ -1186,21 +1238,24 @@ int pp_propagate(pp_ctx* c, const pp_emitter* emitter, const pp_spectrum* spectr
     q.seed = seed;
     q.stream = stream;
     q.max_steps = max_steps;
-    if ((rc = ensure(c->staging, std::max<uint64_t>(hits_cap, 1) * sizeof(pp_hit)))) return rc;
-    if ((rc = enqueue_propagation(c, &q, 1, spectrum, (pp_hit*)c->staging.p, hits_cap, true)))
-        return rc;
+    SpecParams sp;
+    uint64_t n_tiles = 0;
+    if ((rc = plan_launch(c, &q, 1, spectrum, &sp, &n_tiles))) return rc;
+    WorkBufs W;
+    if ((rc = ctx_work(c, 1, n_tiles, hits_cap, sp.kind == PP_SPEC_KNOTS, false, &W))) return rc;
+    if ((rc = enqueue_propagation(c, 1, n_tiles, sp, spectrum, W, hits_cap, true))) return rc;
     if ((rc = ensure_io(c))) return rc;
     char* down = static_cast<char*>(c->io) + kIoDownload;
-    PP_HIP(hipMemcpyAsync(down, c->sums.p, sizeof(pp_summary), hipMemcpyDeviceToHost, c->stream));
-    PP_HIP(hipMemcpyAsync(down + sizeof(pp_summary), (uint32_t*)c->counters.p + 2,
-                          sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    PP_HIP(hipMemcpyAsync(down, W.sums, sizeof(pp_summary), hipMemcpyDeviceToHost, c->stream));
+    PP_HIP(hipMemcpyAsync(down + sizeof(pp_summary), W.counters + 2, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
     PP_HIP(hipStreamSynchronize(c->stream));
     pp_summary s;
     unsigned long long cursor = 0;
     memcpy(&s, down, sizeof s);
     memcpy(&cursor, down + sizeof(pp_summary), sizeof cursor);
     const uint64_t stored = std::min<uint64_t>(cursor, hits_cap);
-    if (stored && (rc = copy_records_to_host(c, hits_out, c->staging.p, stored * sizeof(pp_hit))))
+    if (stored && (rc = copy_records_to_host(c, hits_out, W.staging, stored * sizeof(pp_hit))))
         return rc;
     s.n_stored = stored;
     s.status = s.n_hits > stored ? PP_ERR_OVERFLOW : PP_OK;
@@ -1224,12 +1279,13 @@ int pp_propagate_batch(pp_ctx* c, const pp_request* requests, uint32_t n_req,
     const uint64_t out_cap = out_records(hits_cap, per_request_cap, n_req);
     if ((rc = ensure(c->out_hits, std::max<uint64_t>(out_cap, 1) * sizeof(pp_hit)))) return rc;
     if ((rc = ensure(c->offsets, ((uint64_t)n_req + 1) * sizeof(uint64_t)))) return rc;
+    pp_summary* d_sums = nullptr;
     if ((rc = run_batch(c, requests, n_req, spectrum, (pp_hit*)c->out_hits.p, hits_cap,
-                        per_request_cap, (uint64_t*)c->offsets.p)))
+                        per_request_cap, (uint64_t*)c->offsets.p, &d_sums)))
         return rc;
     PP_HIP(hipMemcpyAsync(offsets_out, c->offsets.p, ((uint64_t)n_req + 1) * sizeof(uint64_t),
                           hipMemcpyDeviceToHost, c->stream));
-    PP_HIP(hipMemcpyAsync(summaries_out, c->sums.p, (uint64_t)n_req * sizeof(pp_summary),
+    PP_HIP(hipMemcpyAsync(summaries_out, d_sums, (uint64_t)n_req * sizeof(pp_summary),
                           hipMemcpyDeviceToHost, c->stream));
     PP_HIP(hipStreamSynchronize(c->stream));
     const uint64_t stored = offsets_out[n_req];
@@ -1250,10 +1306,11 @@ int pp_propagate_batch_device(pp_ctx* c, const pp_request* requests, uint32_t n_
     if (rc) return rc;
     if ((hits_cap && !d_hits) || !d_offsets || !d_summaries)
         return fail(PP_ERR_INVALID_ARG, "device output pointer is NULL");
+    pp_summary* d_sums = nullptr;
     if ((rc = run_batch(c, requests, n_req, spectrum, d_hits, hits_cap, per_request_cap,
-                        d_offsets)))
+                        d_offsets, &d_sums)))
         return rc;
-    PP_HIP(hipMemcpyAsync(d_summaries, c->sums.p, (uint64_t)n_req * sizeof(pp_summary),
+    PP_HIP(hipMemcpyAsync(d_summaries, d_sums, (uint64_t)n_req * sizeof(pp_summary),
                           hipMemcpyDeviceToDevice, c->stream));
     return PP_OK;
 }
@@ -2100,9 +2157,9 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     PP_HIP(hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking));
     PP_HIP(hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking));
     PP_HIP(hipEventCreateWithFlags(&s->kdone, hipEventDisableTiming));
-    // The spectrum's knots go to the context's knot buffer first; keep a copy.
-    if ((rc = prepare_spectrum(c, spectrum, &s->spec))) return rc;
+    if ((rc = check_spectrum(spectrum, &s->spec))) return rc;
     if ((rc = ensure(s->knots, PP_MAX_KNOTS * sizeof(float)))) return rc;
+    if ((rc = upload_knots(spectrum, (float*)s->knots.p, c->stream))) return rc;
     // the server keeps its own copy of the medium table (pp_set_medium on the
     // context does not change requests already being served)
     if ((rc = ensure(s->table, (size_t)c->n_points * sizeof(float4)))) return rc;
@@ -2113,9 +2170,6 @@ int server_setup(pp_server* s, const pp_spectrum* spectrum) {
     s->inv_dl = c->inv_dl;
     s->g = c->g;
     s->n_min = c->n_min;
-    if (s->spec.kind == PP_SPEC_KNOTS)
-        PP_HIP(hipMemcpyAsync(s->knots.p, c->knots.p, s->spec.n_knots * sizeof(float),
-                              hipMemcpyDeviceToDevice, c->stream));
     PP_HIP(hipStreamSynchronize(c->stream));
     const uint32_t S = s->cfg.max_slots;
     PP_HIP(hipHostMalloc((void**)&s->ctl, sizeof(SrvControl), hipHostMallocCoherent));
