@@ -217,6 +217,28 @@ int pp_propagate_batch_device(pp_ctx* ctx, const pp_request* requests, uint32_t 
                               uint64_t per_request_cap, uint64_t* d_offsets,
                               pp_summary* d_summaries);
 
+/* Allocation-free form of pp_propagate_batch_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and bitwise the same summaries and per-request hit
+ * sets; the call itself allocates nothing.  Its work buffers (requests, tile
+ * partials, counters, hit staging, compaction arrays) live in one fixed
+ * device array of PP_NOALLOC_SCRATCH_BYTES per GPU, set up by pp_create and
+ * held until the process ends; the summaries go straight to d_summaries.
+ * Limits, refused with PP_ERR_INVALID_ARG: n_req <= PP_NOALLOC_MAX_REQUESTS,
+ * at most PP_NOALLOC_MAX_TILES tiles of PP_TILE_PHOTONS photons in all,
+ * hits_cap <= PP_NOALLOC_MAX_HITS.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several contexts or threads) race;
+ * serialise them. */
+#define PP_NOALLOC_MAX_REQUESTS 4096u
+#define PP_NOALLOC_MAX_TILES (1u << 20)     /* 8.6e9 photons per call */
+#define PP_NOALLOC_MAX_HITS (1u << 18)      /* staged hit records per call */
+#define PP_NOALLOC_SCRATCH_BYTES 60166656ull /* 57.4 MiB per GPU */
+int pp_propagate_batch_device_noalloc(pp_ctx* ctx, const pp_request* requests, uint32_t n_req,
+                                      const pp_spectrum* spectrum, pp_hit* d_hits,
+                                      uint64_t hits_cap, uint64_t per_request_cap,
+                                      uint64_t* d_offsets, pp_summary* d_summaries);
+
 /* Wait for all work enqueued on the context's stream. */
 int pp_synchronize(pp_ctx* ctx);
 

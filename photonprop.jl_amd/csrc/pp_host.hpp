@@ -91,4 +91,6 @@ struct pp_ctx {
     hipEvent_t copy_ev[2] = {nullptr, nullptr};
     std::vector<pph::DevReq> h_reqs;
     std::vector<uint64_t> h_tiles;
+    // device address of this device's `scratch` array (allocation-free entry point)
+    void* na_scratch = nullptr;
 };

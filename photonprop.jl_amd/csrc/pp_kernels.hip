@@ -1084,6 +1084,75 @@ uint64_t out_records(uint64_t hits_cap, uint64_t per_cap, uint32_t n_req) {
 
 }  // namespace
 
+// ---- work buffers of the allocation-free entry point ------------------------
+// pp_propagate_batch_device_noalloc takes every work buffer from one
+// file-scope device array, carved into fixed regions (offsets below, each
+// 256-byte aligned); the summaries go straight to the caller's array.  Every
+// GPU the library runs on holds its own copy until the process ends.  The
+// array's name is a host-side stand-in: host code uses only the address that
+// hipGetSymbolAddress returns for the current device (na_scratch_setup).
+namespace {
+
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t kNaReqs = 0;
+constexpr uint64_t kNaTileStart = kNaReqs + na_align(sizeof(DevReq) * PP_NOALLOC_MAX_REQUESTS);
+constexpr uint64_t kNaPartials = kNaTileStart + na_align(8ull * (PP_NOALLOC_MAX_REQUESTS + 1));
+constexpr uint64_t kNaCounters = kNaPartials + na_align(sizeof(TilePartial) * (uint64_t)PP_NOALLOC_MAX_TILES);
+constexpr uint64_t kNaKnots = kNaCounters + 256;
+constexpr uint64_t kNaStaging = kNaKnots + na_align(sizeof(float) * PP_MAX_KNOTS);
+constexpr uint64_t kNaCnt = kNaStaging + na_align(sizeof(pp_hit) * (uint64_t)PP_NOALLOC_MAX_HITS);
+constexpr uint64_t kNaFill = kNaCnt + na_align(4ull * PP_NOALLOC_MAX_REQUESTS);
+constexpr uint64_t kNaRecReq = kNaFill + na_align(4ull * PP_NOALLOC_MAX_REQUESTS);
+constexpr uint64_t kNaBytes = kNaRecReq + na_align(4ull * PP_NOALLOC_MAX_HITS);
+static_assert(kNaBytes == PP_NOALLOC_SCRATCH_BYTES, "scratch layout and the header's stated size differ");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};  // device address of `scratch`, per device
+
+// Once per device, with that device current: the device address of its copy
+// of `scratch`.  Devices beyond kNaMaxDevices get none (the entry point then
+// refuses).
+int na_scratch_setup(pp_ctx* c) {
+    if (c->device >= kNaMaxDevices) return PP_OK;
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (!g_na_ptr[c->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PP_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", c->device,
+                        hipGetErrorString(e));
+        g_na_ptr[c->device] = p;
+    }
+    c->na_scratch = g_na_ptr[c->device];
+    return PP_OK;
+}
+
+// The allocation-free entry point's work buffers in the device's scratch array.
+WorkBufs scratch_work(void* base, pp_summary* d_summaries) {
+    char* b = static_cast<char*>(base);
+    WorkBufs W;
+    W.reqs = reinterpret_cast<DevReq*>(b + kNaReqs);
+    W.tile_start = reinterpret_cast<uint64_t*>(b + kNaTileStart);
+    W.partials = reinterpret_cast<TilePartial*>(b + kNaPartials);
+    W.counters = reinterpret_cast<uint32_t*>(b + kNaCounters);
+    W.sums = d_summaries;
+    W.knots = reinterpret_cast<float*>(b + kNaKnots);
+    W.staging = reinterpret_cast<pp_hit*>(b + kNaStaging);
+    W.cnt = reinterpret_cast<uint32_t*>(b + kNaCnt);
+    W.fill = reinterpret_cast<uint32_t*>(b + kNaFill);
+    W.rec_req = reinterpret_cast<uint32_t*>(b + kNaRecReq);
+    return W;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* pp_last_error(void) { return last_error().c_str(); }
@@ -1115,7 +1184,11 @@ int pp_create(pp_ctx** out, int device, const pp_medium_tables* medium) {
         pp_destroy(c);
         return fail(PP_ERR_DEVICE, "stream/event creation: %s", hipGetErrorString(e));
     }
-    if ((rc = upload_medium(c, medium))) {
+    // host request records of up to PP_NOALLOC_MAX_REQUESTS requests without
+    // growing later (the allocation-free entry point's bound)
+    c->h_reqs.reserve(PP_NOALLOC_MAX_REQUESTS);
+    c->h_tiles.reserve(PP_NOALLOC_MAX_REQUESTS + 1);
+    if ((rc = upload_medium(c, medium)) || (rc = na_scratch_setup(c))) {
         std::string msg = last_error();
         pp_destroy(c);
         last_error() = msg;
@@ -1313,6 +1386,33 @@ int pp_propagate_batch_device(pp_ctx* c, const pp_request* requests, uint32_t n_
     PP_HIP(hipMemcpyAsync(d_summaries, d_sums, (uint64_t)n_req * sizeof(pp_summary),
                           hipMemcpyDeviceToDevice, c->stream));
     return PP_OK;
+}
+
+int pp_propagate_batch_device_noalloc(pp_ctx* c, const pp_request* requests, uint32_t n_req,
+                                      const pp_spectrum* spectrum, pp_hit* d_hits,
+                                      uint64_t hits_cap, uint64_t per_request_cap,
+                                      uint64_t* d_offsets, pp_summary* d_summaries) {
+    int rc = validate_ctx(c);
+    if (rc) return rc;
+    if ((hits_cap && !d_hits) || !d_offsets || !d_summaries)
+        return fail(PP_ERR_INVALID_ARG, "device output pointer is NULL");
+    if (!c->na_scratch)
+        return fail(PP_ERR_DEVICE, "no scratch array on device %d", c->device);
+    if (n_req > PP_NOALLOC_MAX_REQUESTS)
+        return fail(PP_ERR_INVALID_ARG, "n_req %u exceeds PP_NOALLOC_MAX_REQUESTS (%u)", n_req,
+                    (unsigned)PP_NOALLOC_MAX_REQUESTS);
+    if (hits_cap > PP_NOALLOC_MAX_HITS)
+        return fail(PP_ERR_INVALID_ARG, "hits_cap %llu exceeds PP_NOALLOC_MAX_HITS (%u)",
+                    (unsigned long long)hits_cap, (unsigned)PP_NOALLOC_MAX_HITS);
+    SpecParams sp;
+    uint64_t n_tiles = 0;
+    if ((rc = plan_launch(c, requests, n_req, spectrum, &sp, &n_tiles))) return rc;
+    if (n_tiles > PP_NOALLOC_MAX_TILES)
+        return fail(PP_ERR_INVALID_ARG, "%llu tiles of %d photons exceed PP_NOALLOC_MAX_TILES (%u)",
+                    (unsigned long long)n_tiles, PP_TILE_PHOTONS, (unsigned)PP_NOALLOC_MAX_TILES);
+    const WorkBufs W = scratch_work(c->na_scratch, d_summaries);
+    if ((rc = enqueue_propagation(c, n_req, n_tiles, sp, spectrum, W, hits_cap))) return rc;
+    return enqueue_compaction(c, n_req, W, hits_cap, per_request_cap, d_hits, d_offsets);
 }
 
 int pp_synchronize(pp_ctx* c) {

@@ -119,6 +119,9 @@ class pp_server_stats(ctypes.Structure):
 
 PP_TILE_PHOTONS = 8192
 PP_MAX_HIT_RECORDS = 1 << 36
+PP_NOALLOC_MAX_REQUESTS = 4096
+PP_NOALLOC_MAX_TILES = 1 << 20
+PP_NOALLOC_MAX_HITS = 1 << 18
 
 # pp_hit as a numpy record (64 bytes).
 HIT_DTYPE = np.dtype([
@@ -160,6 +163,9 @@ SIGNATURES = {
     "pp_propagate_batch_device": (ctypes.c_int, [_vp, ctypes.POINTER(pp_request), _u32,
                                                  ctypes.POINTER(pp_spectrum), _vp, _u64, _u64, _vp,
                                                  _vp]),
+    "pp_propagate_batch_device_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(pp_request), _u32,
+                                                         ctypes.POINTER(pp_spectrum), _vp, _u64, _u64,
+                                                         _vp, _vp]),
     "pp_synchronize": (ctypes.c_int, [_vp]),
     "pp_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "pp_server_create": (ctypes.c_int, [_vp, ctypes.POINTER(pp_spectrum), ctypes.POINTER(pp_server_config),
