@@ -44,9 +44,22 @@ def workload_requests(pp, photons_per_geometry, seed=0):
     return workloads.scale_geometries(photons=photons_per_geometry, radius=0.21, seed=seed)
 
 
-def cpu_baseline(pp, photons_per_geometry_sample):
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pp, photons_per_geometry_sample, runs=5):
     """The oracle (FP32, OpenMP over all host cores this process may use) on
-    the same 64 geometries with a bounded photon sample per geometry."""
+    the same 64 geometries with a bounded photon sample per geometry: one
+    warm-up run, then the median of `runs` timed runs."""
     import oracle
     from oracle.compare import oracle_for
 
@@ -54,25 +67,39 @@ def cpu_baseline(pp, photons_per_geometry_sample):
     reqs = workload_requests(pp, photons_per_geometry_sample)
     threads = len(os.sched_getaffinity(0))
     oracle.lib()
-    steps = hits = 0
-    t0 = time.perf_counter()
-    for r in reqs:
-        src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0, int(r["n_photons"]),
-                              pp.InverseSquareSpectrum(300.0, 600.0))
-        tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
-        _, s = oracle_for(tables, src, tgt, seed=int(r["seed"]), stream=int(r["stream"]),
-                          n_threads=threads)
-        steps += s["n_steps"]
-        hits += s["n_hits"]
-    dt = time.perf_counter() - t0
+
+    def one_run():
+        steps = hits = 0
+        t0 = time.perf_counter()
+        for r in reqs:
+            src = pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0, int(r["n_photons"]),
+                                  pp.InverseSquareSpectrum(300.0, 600.0))
+            tgt = pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))
+            _, s = oracle_for(tables, src, tgt, seed=int(r["seed"]), stream=int(r["stream"]),
+                              n_threads=threads)
+            steps += s["n_steps"]
+            hits += s["n_hits"]
+        return steps, hits, time.perf_counter() - t0
+
+    one_run()  # warm-up
+    results = [one_run() for _ in range(runs)]
+    steps, hits = results[0][0], results[0][1]
+    assert all(r[0] == steps and r[1] == hits for r in results)  # same work every run
+    times = sorted(r[2] for r in results)
+    dt = times[len(times) // 2]
     return {
         "value": steps / dt,
         "unit": "photon steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": "64 geometries x %d photons (%d steps, %d hits) in %.1f s; detected photons/s %.4g"
-                  % (photons_per_geometry_sample, steps, hits, dt, hits / dt),
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "sample": "64 geometries x %d photons (%d steps, %d hits): median of %d runs after one "
+                  "warm-up, %.2f s (runs %s s); detected photons/s %.4g"
+                  % (photons_per_geometry_sample, steps, hits, runs, dt,
+                     ", ".join("%.2f" % t for t in times), hits / dt),
         "detected_photons_per_s": hits / dt,
+        "steps_per_s_per_core": steps / dt / threads,
     }
 
 
@@ -302,8 +329,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e9, help="photons per geometry")
-    ap.add_argument("--cpu-sample", type=float, default=1e7,
-                    help="photons per geometry for the CPU baseline (0 skips it)")
+    ap.add_argument("--cpu-sample", type=float, default=2e6,
+                    help="photons per geometry for the CPU baseline, timed as the median of 5 "
+                         "runs after a warm-up (0 skips it)")
     ap.add_argument("--extras", type=int, default=1,
                     help="also measure the 4096-pair batch and the serving loop (0 skips)")
     ap.add_argument("--serve-requests", type=int, default=1000,

@@ -118,6 +118,19 @@ struct KParams {
 // Propagation kernel
 // ---------------------------------------------------------------------------
 
+// Element i of data that no kernel writes during the launch, read through the
+// constant address space: a uniform address then loads with scalar loads into
+// SGPRs (through the scalar cache) instead of vector loads into VGPRs.  (The
+// host pass of this file only parses device code, and has no address space 4.)
+template <typename T>
+__device__ __forceinline__ T load_constant(const T* p, uint64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+    return p[i];
+#endif
+}
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -139,6 +152,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 // 1,135 ms.  64 records (16 KiB per block) also always take a whole round.
 constexpr uint32_t kHitBuf = 64;
 static_assert(kHitBuf >= 64, "a round's hits (up to 64) must fit an empty buffer");
+static_assert((1ull << 32) % kTilePhotons == 0, "a tile never straddles photon id 2^32");
 
 // 16 B of a record: plain, or (serving) as system-scope 32-bit stores that go
 // straight to memory, where the host copies records while the kernel runs.
@@ -241,8 +255,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     for (uint32_t k = 0; k < rounds; ++k) {
         const uint32_t local = k * kBlock + tid;
         const bool active = local < count;
-        const uint64_t pid = first + local;
-        const uint32_t id_lo = (uint32_t)pid, id_hi = (uint32_t)(pid >> 32);
+        // a tile's photons share the high word of their id: `first` is a
+        // multiple of kTilePhotons, which divides 2^32, and local < kTilePhotons
+        const uint32_t id_lo = (uint32_t)first + local, id_hi = (uint32_t)(first >> 32);
 
         // ---- emission (src/photon_prop_cuda.jl:308-316) ----
         // Draws (ppd::draw_a, ppd::draw_p) per Philox block (x, y, z, w):
@@ -303,7 +318,10 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         uint32_t nst = 0u;
 
         // One scattering-loop iteration (src/photon_prop_cuda.jl:320-376, D3).
-#define PP_STEP(K, AS, AC, PH, RN)                                                       \
+        // LAST (wave-uniform): the photon's final allowed step.  A photon that
+        // does not hit there ends with it, so its new direction would never be
+        // used and the scatter is skipped; results are unchanged.
+#define PP_STEP(K, AS, AC, PH, RN, LAST)                                                 \
     if (alive) {                                                                        \
         ++nst;                                                                          \
         float s = -__builtin_amdgcn_logf(2.0f - (AS)) * sca_ln2;                        \
@@ -327,7 +345,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         if (h) {                                                                        \
             hit = true;                                                                 \
             alive = false;                                                              \
-        } else {                                                                        \
+        } else if (!(LAST)) {                                                           \
             const float ct = ppd::hg_costheta_a_t<kIso>(T.hg, (AC));                    \
             const float st = ppd::sin_from_cos(ct);                                     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
@@ -341,20 +359,25 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         // The direction is renormalised after step 0 and after steps 4, 8, ...
         // (at most four rotations apart; the emission direction is left to
         // step 0).  Two Philox blocks per iteration, one per pair of steps.
-        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), true)
+        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), true,
+                max_steps == 1u)
         for (uint32_t s0 = 1; s0 < max_steps; s0 += 4) {
             ppd::U4 Q;
             if (alive)
                 Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false)
+            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false,
+                    s0 + 1 >= max_steps)
             if (s0 + 1 >= max_steps) break;
-            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), false)
+            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), false,
+                    s0 + 2 >= max_steps)
             if (s0 + 2 >= max_steps) break;
             if (alive)
                 Q = ppd::philox10((s0 + 3u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
-            PP_STEP(s0 + 2u, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false)
+            PP_STEP(s0 + 2u, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false,
+                    s0 + 3 >= max_steps)
             if (s0 + 3 >= max_steps) break;
-            PP_STEP(s0 + 3u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), true)
+            PP_STEP(s0 + 3u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), true,
+                    s0 + 4 >= max_steps)
         }
 #undef PP_STEP
         acc_steps += nst;
@@ -464,11 +487,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         uint32_t lo = 0, hi = P.n_req;  // invariant: tile_start[lo] <= tile < tile_start[hi]
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (P.tile_start[mid] <= (uint64_t)tile) lo = mid; else hi = mid;
+            if (load_constant(P.tile_start, mid) <= (uint64_t)tile) lo = mid; else hi = mid;
         }
         const uint32_t r = __builtin_amdgcn_readfirstlane(lo);
-        const DevReq R = P.reqs[r];
-        const uint64_t first = ((uint64_t)tile - P.tile_start[r]) * kTilePhotons;
+        // through the constant address space: the request is not written
+        // during the launch, so it is read with scalar loads into SGPRs
+        const DevReq R = load_constant(P.reqs, r);
+        const uint64_t first = ((uint64_t)tile - load_constant(P.tile_start, r)) * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
         const TilePartial tp = run_tile_any<false>(T, R, r, first, count, P.hits, P.hits_cap,
