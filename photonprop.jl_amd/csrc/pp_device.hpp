@@ -31,12 +31,17 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 
 // Philox4x32-10.  The key (k0, k1) is wave-uniform, so its schedule lives in
 // scalar registers; each round costs two 32x32->64 multiplies (v_mad_u64_u32)
-// and two 3-input XORs per lane.  Where a round's XOR has two operands that
-// depend only on uniform counter parts (block index, stream) and the key --
-// words 0 and 2 of round 0, word 2 of round 1 and word 0 of round 2 (counting
-// from 0) -- it is written as plain XORs, so that the compiler keeps those
-// words, and their multiplies, in scalar registers; v_bitop3_b32 would pull
-// them into the vector unit.
+// and two 3-input XORs per lane.  The propagation calls it with the counter
+// words c0 (block index), c2 (high photon-id word) and c3 (stream) uniform
+// and only c1 (low photon-id word) varying, so in the first rounds some words
+// and their multiplies stay uniform:
+//   round 0: only word 0 varies (c1 ^ uniform);
+//   round 1: words 2, 3 vary; word 0 is uniform again;
+//   round 2: words 0, 1, 2 vary; round 3 on: all four.
+// Each XOR is grouped so that its uniform operands combine first (one scalar
+// XOR, then at most one vector XOR); a v_bitop3_b32 would take only one
+// scalar operand and need a v_mov for the other.  The grouping is only
+// associativity, so any counter (uniform or not) gives the same words.
 __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                        uint32_t k0, uint32_t k1) {
     // opaque key: each round's key is one scalar add from the last, instead of
@@ -49,8 +54,23 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
         const uint64_t p0 = (uint64_t)kM0 * c0;
         const uint64_t p1 = (uint64_t)kM1 * c2;
         const uint32_t h1 = (uint32_t)(p1 >> 32), h0 = (uint32_t)(p0 >> 32);
-        const uint32_t n0 = r < 3 ? h1 ^ c1 ^ k0 : xor3(h1, c1, k0);
-        const uint32_t n2 = r < 2 ? h0 ^ c3 ^ k1 : xor3(h0, c3, k1);
+        uint32_t n0, n2;
+        if (r == 0) {
+            n0 = c1 ^ (h1 ^ k0);
+            n2 = (h0 ^ k1) ^ c3;
+        } else if (r == 1) {
+            n0 = (h1 ^ k0) ^ c1;
+            n2 = h0 ^ (c3 ^ k1);
+        } else if (r == 2) {
+            n0 = h1 ^ (c1 ^ k0);
+            n2 = c3 ^ (h0 ^ k1);
+        } else if (r == 3) {
+            n0 = xor3(h1, c1, k0);
+            n2 = h0 ^ (c3 ^ k1);
+        } else {
+            n0 = xor3(h1, c1, k0);
+            n2 = xor3(h0, c3, k1);
+        }
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -146,10 +166,14 @@ __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
 // transcendental.  kRenorm = false skips it: one rotation of a unit vector
 // moves |r| by ~2e-7, so a caller may renormalise every few rotations (after
 // four, |e| stays below ~1e-6 and the Newton step is still exact to ~1e-12).
-template <bool kRenorm = true>
+// kInBiased: the caller's y already carries the -2^-60; kOutBiased: the
+// result's y carries it too, folded into its first fma (an fmaak with the
+// literal), so a loop that feeds each result into the next rotation pays for
+// the bias with no instruction.
+template <bool kRenorm = true, bool kInBiased = false, bool kOutBiased = false>
 __device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, float nx, float ny,
                                                float nz, float& ox, float& oy, float& oz) {
-    const float y = y_in - 0x1p-60f;
+    const float y = kInBiased ? y_in : y_in - 0x1p-60f;
     const float s2 = __builtin_fmaf(y, y, x * x);
     const float zp = 1.0f + z;
     const float inv_h = __builtin_amdgcn_rcpf(__builtin_fmaf(zp, zp, s2)) * 2.0f;
@@ -157,7 +181,8 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, flo
     const float q = cr * inv_h;
     const float dxy = __builtin_fmaf(x, nx, y * ny);
     const float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
-    const float ry = __builtin_fmaf(ny, z, __builtin_fmaf(y, nz, x * q));
+    const float ry = __builtin_fmaf(
+        ny, z, __builtin_fmaf(y, nz, kOutBiased ? __builtin_fmaf(x, q, -0x1p-60f) : x * q));
     const float rz = __builtin_fmaf(nz, z, -dxy);
     if (!kRenorm) {
         ox = rx;

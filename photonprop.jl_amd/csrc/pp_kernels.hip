@@ -294,7 +294,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                 const float st = __builtin_amdgcn_sinf(theta * kInv2Pi);
                 const float ct = __builtin_amdgcn_cosf(theta * kInv2Pi);
                 dx = cp * st;
-                dy = sp * st;
+                dy = __builtin_fmaf(sp, st, -0x1p-60f);  // the step loop's y bias
                 dz = ct;
             } else {
                 const float ct =
@@ -302,11 +302,14 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                         ? __builtin_fmaf(2.0f, u_th, -1.0f)
                         : fminf(__builtin_amdgcn_rcpf(n_phase * R.beta), 1.0f);
                 const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
-                // not renormalised here: step 0's rotation renormalises
-                ppd::rotate_to_axis<false>(R.axis[0], R.axis[1], R.axis[2], cp * st, sp * st,
-                                           ct, dx, dy, dz);
+                // not renormalised here: step 0's rotation renormalises; y
+                // carries the step loop's bias
+                ppd::rotate_to_axis<false, false, true>(R.axis[0], R.axis[1], R.axis[2], cp * st,
+                                                        sp * st, ct, dx, dy, dz);
             }
         }
+        // (dy carries a bias of -2^-60, which the rotations need at the exact
+        // south pole; it leaves every |y| above 2^-36 unchanged)
         const float idx = dx, idy = dy, idz = dz;
         // position relative to the target centre: the sphere test needs no
         // per-step subtraction; the absolute position is rebuilt for a hit
@@ -350,7 +353,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             const float st = ppd::sin_from_cos(ct);                                     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
-            ppd::rotate_to_axis<RN>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz);    \
+            ppd::rotate_to_axis<RN, true, true>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz); \
         }                                                                               \
     }
 
