@@ -182,7 +182,7 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, flo
     const float dxy = __builtin_fmaf(x, nx, y * ny);
     const float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
     const float ry = __builtin_fmaf(
-        ny, z, __builtin_fmaf(y, nz, kOutBiased ? __builtin_fmaf(x, q, -0x1p-60f) : x * q));
+        ny, z, __builtin_fmaf(x, q, kOutBiased ? __builtin_fmaf(y, nz, -0x1p-60f) : y * nz));
     const float rz = __builtin_fmaf(nz, z, -dxy);
     if (!kRenorm) {
         ox = rx;
