@@ -262,12 +262,22 @@ int pp_last_kernel_ms(pp_ctx* ctx, float* ms);
  * PP_ERR_INVALID_ARG rather than wrapped in a size computation. */
 #define PP_MAX_HIT_RECORDS (1ull << 36)
 
+/* pp_server_config.flags */
+#define PP_SERVER_HOST_POOL 1u  /* hit pool in pinned host memory: the kernel writes records
+                                   straight to the host and results are copied out by the
+                                   host alone.  Used by itself when the device has no SDMA
+                                   engine to copy results from device memory. */
+
 typedef struct pp_server_config {
     uint32_t max_slots;             /* requests on the device at once, 1..4096 */
     uint32_t max_tiles_per_request; /* photons per request <= this * PP_TILE_PHOTONS */
-    uint64_t hit_pool_records;      /* device hit-record pool shared by admitted requests */
-    uint32_t idle_exit_us;          /* persistent kernel exits after this long without work
-                                       (relaunched on demand); 0 -> 20000 */
+    uint64_t hit_pool_records;      /* hit-record pool shared by admitted requests */
+    uint32_t idle_exit_us;          /* the host stops the persistent kernel after this long
+                                       without work (relaunched on demand); 0 -> 20000 */
+    uint32_t device_wait_us;        /* the kernel stops by itself after waiting this long
+                                       (device clock) with nothing published, should the
+                                       host not stop it; 0 -> max(4 * idle_exit_us, 1 s) */
+    uint32_t flags;                 /* PP_SERVER_* */
     uint32_t reserved;
 } pp_server_config;
 
@@ -277,6 +287,9 @@ typedef struct pp_server_stats {
     uint64_t pool_records_free;
     uint64_t tiles_published;
     uint64_t grid_blocks;           /* persistent-kernel blocks (one per resident slot) */
+    uint64_t device_stops;          /* kernel exits after device_wait_us without work */
+    uint64_t kernel_running;        /* 1 while the persistent kernel runs */
+    uint64_t host_pool;             /* 1 if the hit pool is in host memory */
 } pp_server_stats;
 
 typedef struct pp_server pp_server;

@@ -12,15 +12,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/photonprop.h"
+#include "pp_types.hpp"
 
 namespace pph {
-
-// Message of the last failure on this thread (pp_last_error).
-std::string& last_error();
-
-// Record a formatted message and return `code`.
-int fail(int code, const char* fmt, ...);
 
 #define PP_HIP(call)                                                                      \
     do {                                                                                  \
@@ -47,20 +41,6 @@ inline int ensure(DevBuf& b, size_t bytes) {
     b.cap = want;
     return PP_OK;
 }
-
-// Request as the kernel reads it (host-precomputed).
-struct DevReq {
-    float pos[3];
-    float t0;
-    float axis[3];
-    float beta;
-    float center[3];
-    float r2;
-    uint64_t n_photons;
-    uint32_t profile;
-    uint32_t max_steps;
-    uint32_t seed_lo, seed_hi, stream, pad;
-};
 
 }  // namespace pph
 

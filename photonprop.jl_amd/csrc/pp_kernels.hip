@@ -46,34 +46,16 @@
 #include "../../include/photonprop.h"
 #include "pp_device.hpp"
 #include "pp_host.hpp"
+#include "pp_serve_core.hpp"
 
 using namespace pph;
-
-namespace pph {
-
-std::string& last_error() {
-    thread_local std::string msg;
-    return msg;
-}
-
-int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    last_error() = buf;
-    return code;
-}
-
-}  // namespace pph
 
 namespace {
 
 constexpr int kBlock = 256;                       // lanes per block (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kRoundsPerTile = 32;            // photons per lane per tile
-constexpr uint64_t kTilePhotons = (uint64_t)kBlock * kRoundsPerTile;  // 8192
+static_assert((uint64_t)kBlock * kRoundsPerTile == kTilePhotons, "a tile is kRoundsPerTile rounds of a block");
 constexpr double kCVacuum = 0.299792458;           // m/ns
 constexpr float kLn2 = 0.69314718055994530942f;
 constexpr float kLog2e = 1.44269504088896340736f;
@@ -84,15 +66,6 @@ struct TilePartial {
     uint64_t n_steps;
     uint64_t n_hits;
     double sum_w, sum_wt, sum_wt2;
-};
-
-struct SpecParams {
-    uint32_t kind;
-    uint32_t n_knots;
-    float a;          // INV_LAMBDA2: 1/lo
-    float d;          // INV_LAMBDA2: 1/lo - 1/hi
-    float mono;
-    float knot_step;  // KNOTS: 1/(n_knots-1)
 };
 
 struct KParams {
@@ -759,26 +732,6 @@ __global__ void __launch_bounds__(256) selftest_hg_moments_kernel(ppd::HG hg, ui
 
 namespace {
 
-int check_medium(const pp_medium_tables* m) {
-    if (!m) return fail(PP_ERR_INVALID_ARG, "medium is NULL");
-    if (m->n_points < 2 || m->n_points > PP_MAX_TABLE_POINTS)
-        return fail(PP_ERR_INVALID_ARG, "medium n_points=%u outside [2, %d]", m->n_points,
-                    PP_MAX_TABLE_POINTS);
-    if (!(m->lambda_max > m->lambda_min) || !std::isfinite(m->lambda_min) ||
-        !std::isfinite(m->lambda_max))
-        return fail(PP_ERR_INVALID_ARG, "medium wavelength range [%g, %g] invalid",
-                    (double)m->lambda_min, (double)m->lambda_max);
-    if (!m->n_phase || !m->abs_len || !m->sca_len)
-        return fail(PP_ERR_INVALID_ARG, "medium table pointer is NULL");
-    if (!(std::fabs(m->g) < 1.0f)) return fail(PP_ERR_INVALID_ARG, "medium g=%g not in (-1,1)", (double)m->g);
-    for (uint32_t i = 0; i < m->n_points; ++i) {
-        if (!(m->abs_len[i] > 0.0f) || !(m->sca_len[i] > 0.0f) || !(m->n_phase[i] > 0.0f) ||
-            !std::isfinite(m->abs_len[i]) || !std::isfinite(m->sca_len[i]) ||
-            !std::isfinite(m->n_phase[i]))
-            return fail(PP_ERR_INVALID_ARG, "medium table entry %u not positive and finite", i);
-    }
-    return PP_OK;
-}
 
 int upload_medium(pp_ctx* c, const pp_medium_tables* m) {
     int rc = check_medium(m);
@@ -797,39 +750,6 @@ int upload_medium(pp_ctx* c, const pp_medium_tables* m) {
     return PP_OK;
 }
 
-// Validate a spectrum and fill the kernel parameters (host only).
-int check_spectrum(const pp_spectrum* s, SpecParams* out) {
-    if (!s) return fail(PP_ERR_INVALID_ARG, "spectrum is NULL");
-    memset(out, 0, sizeof *out);
-    out->kind = s->kind;
-    switch (s->kind) {
-    case PP_SPEC_MONO:
-        if (!(s->mono > 0.0f) || !std::isfinite(s->mono))
-            return fail(PP_ERR_INVALID_ARG, "monochromatic wavelength %g invalid", (double)s->mono);
-        out->mono = s->mono;
-        return PP_OK;
-    case PP_SPEC_INV_LAMBDA2:
-        if (!(s->lo > 0.0f) || !(s->hi > s->lo) || !std::isfinite(s->hi))
-            return fail(PP_ERR_INVALID_ARG, "spectrum range [%g, %g] invalid", (double)s->lo,
-                        (double)s->hi);
-        out->a = (float)(1.0 / (double)s->lo);
-        out->d = (float)(1.0 / (double)s->lo - 1.0 / (double)s->hi);
-        return PP_OK;
-    case PP_SPEC_KNOTS: {
-        if (s->n_knots < 2 || s->n_knots > PP_MAX_KNOTS || !s->knots)
-            return fail(PP_ERR_INVALID_ARG, "spectrum knots: n_knots=%u (need 2..%d) or NULL",
-                        s->n_knots, PP_MAX_KNOTS);
-        for (uint32_t i = 0; i < s->n_knots; ++i)
-            if (!(s->knots[i] > 0.0f) || !std::isfinite(s->knots[i]))
-                return fail(PP_ERR_INVALID_ARG, "spectrum knot %u = %g invalid", i, (double)s->knots[i]);
-        out->n_knots = s->n_knots;
-        out->knot_step = 1.0f / (float)(s->n_knots - 1);
-        return PP_OK;
-    }
-    default:
-        return fail(PP_ERR_UNSUPPORTED, "Cannot initialize spectrum kind %u", s->kind);
-    }
-}
 
 // A checked spectrum's knots to d_knots (PP_MAX_KNOTS floats) on `stream`;
 // nothing for the other kinds.
@@ -840,57 +760,7 @@ int upload_knots(const pp_spectrum* s, float* d_knots, hipStream_t stream) {
     return PP_OK;
 }
 
-int make_devreq(const pp_request* q, uint32_t idx, DevReq* d) {
-    const pp_emitter& e = q->emitter;
-    if (e.profile > PP_EMIT_REF_ISOTROPIC)
-        return fail(PP_ERR_UNSUPPORTED, "request %u: Cannot initialize emission profile %u", idx,
-                    e.profile);
-    for (int j = 0; j < 3; ++j)
-        if (!std::isfinite(e.pos[j]) || !std::isfinite(e.axis[j]) ||
-            !std::isfinite(q->target.center[j]))
-            return fail(PP_ERR_INVALID_ARG, "request %u: non-finite position/axis/centre", idx);
-    if (!std::isfinite(e.t0)) return fail(PP_ERR_INVALID_ARG, "request %u: non-finite t0", idx);
-    if (!(q->target.radius > 0.0f) || !std::isfinite(q->target.radius))
-        return fail(PP_ERR_INVALID_ARG, "request %u: target radius %g must be > 0", idx,
-                    (double)q->target.radius);
-    const double an = std::sqrt((double)e.axis[0] * e.axis[0] + (double)e.axis[1] * e.axis[1] +
-                                (double)e.axis[2] * e.axis[2]);
-    if (!(an > 0.0)) return fail(PP_ERR_INVALID_ARG, "request %u: zero emitter axis", idx);
-    if (e.profile == PP_EMIT_CHERENKOV_CONE && !(e.beta > 0.0f && std::isfinite(e.beta)))
-        return fail(PP_ERR_INVALID_ARG, "request %u: beta %g must be > 0", idx, (double)e.beta);
-    if (q->max_steps == 0) return fail(PP_ERR_INVALID_ARG, "request %u: max_steps must be >= 1", idx);
-    if (q->max_steps > PP_MAX_STEPS)
-        return fail(PP_ERR_INVALID_ARG, "request %u: max_steps %u too large", idx, q->max_steps);
-    memset(d, 0, sizeof *d);
-    for (int j = 0; j < 3; ++j) {
-        d->pos[j] = e.pos[j];
-        d->axis[j] = (float)(e.axis[j] / an);
-        d->center[j] = q->target.center[j];
-    }
-    d->t0 = e.t0;
-    d->beta = e.beta;
-    d->r2 = q->target.radius * q->target.radius;
-    d->n_photons = e.n_photons;
-    d->profile = e.profile;
-    d->max_steps = q->max_steps;
-    d->seed_lo = (uint32_t)q->seed;
-    d->seed_hi = (uint32_t)(q->seed >> 32);
-    d->stream = q->stream;
-    return PP_OK;
-}
 
-// Cherenkov cone needs n(λ)·β >= 1 everywhere on the table.
-int check_cone(const pp_ctx* c, const pp_request* reqs, uint32_t n_req, float n_min) {
-    for (uint32_t i = 0; i < n_req; ++i) {
-        if (reqs[i].emitter.profile != PP_EMIT_CHERENKOV_CONE) continue;
-        if (!((double)n_min * (double)reqs[i].emitter.beta >= 1.0 - 1e-6))
-            return fail(PP_ERR_INVALID_ARG,
-                        "request %u: no Cherenkov emission (n_min=%g, beta=%g, n*beta < 1)", i,
-                        (double)n_min, (double)reqs[i].emitter.beta);
-    }
-    (void)c;
-    return PP_OK;
-}
 
 
 // Henyey-Greenstein constants for the device: 1-g², 1+g², 1/(2g).
@@ -950,7 +820,7 @@ int plan_launch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spec
     if (!reqs) return fail(PP_ERR_INVALID_ARG, "requests is NULL");
     int rc = check_spectrum(spec, sp);
     if (rc) return rc;
-    if ((rc = check_cone(c, reqs, n_req, c->n_min))) return rc;
+    if ((rc = check_cone(reqs, n_req, c->n_min))) return rc;
     c->h_reqs.resize(n_req);
     c->h_tiles.resize((size_t)n_req + 1);
     c->h_tiles[0] = 0;
@@ -1079,12 +949,6 @@ int validate_ctx(pp_ctx* c) {
     return PP_OK;
 }
 
-int check_records(uint64_t n, const char* what) {
-    if (n > PP_MAX_HIT_RECORDS)
-        return fail(PP_ERR_INVALID_ARG, "%s %llu exceeds PP_MAX_HIT_RECORDS (%llu)", what,
-                    (unsigned long long)n, (unsigned long long)PP_MAX_HIT_RECORDS);
-    return PP_OK;
-}
 
 // Propagation and compaction of a batch into d_out / d_offsets, summaries in
 // the context's buffer (returned in *sums).
@@ -1103,12 +967,6 @@ int run_batch(pp_ctx* c, const pp_request* reqs, uint32_t n_req, const pp_spectr
     return enqueue_compaction(c, n_req, W, hits_cap, per_cap, d_out, d_offsets);
 }
 
-// Records the compacted output of a batch can hold.
-uint64_t out_records(uint64_t hits_cap, uint64_t per_cap, uint32_t n_req) {
-    if (!per_cap) return hits_cap;
-    const unsigned __int128 all = (unsigned __int128)per_cap * n_req;  // no wrap-around
-    return all < hits_cap ? (uint64_t)all : hits_cap;
-}
 
 }  // namespace
 
@@ -1541,83 +1399,56 @@ int pp_selftest_hg_moments(pp_ctx* c, float g, uint64_t seed, uint64_t n, double
 // Serving loop: continuous batching into a persistent kernel
 // ===========================================================================
 //
-// Host side (worker thread): pending FIFO -> admission when a slot and
-// hits_cap records of the device hit pool are free -> the request's slot
-// descriptor and a range entry [t0, t1) of the global tile sequence are
-// written to pinned host memory -> `published` advances past t1.
-// Device side (persistent kernel, every resident block slot filled): a block
-// takes a ticket (the next global tile index) with one atomicAdd and waits
-// until the tile is published.  The published count is mirrored from pinned
-// memory into a device-memory gate word; only the block holding the lowest
-// unpublished ticket (the head) reads the host's counters over PCIe, so the
-// link carries one poller however many blocks wait.  Before it moves the gate
-// the head also copies the newly published range entries into a device-memory
-// ring, so a block finds its range entry by a 64-ary search in device memory
-// (the per-tile reads over PCIe are one 16-byte-line slot descriptor: a
-// search over pinned memory, some 70 scattered reads a tile, had held the
-// serving kernel to 40 % of the batch kernel's throughput on the same work).
-// It runs the tile with the same run_tile as the
-// batch kernel, stores the tile partial, and the block that completes a
-// request's last tile adds the partials in tile order and posts the summary
-// to pinned memory.  The host stops the kernel (`stop`, raised into the gate
-// by the head) only when every published tile is done, so the tickets held at
-// exit are all unpublished and the next launch restarts the ticket counter at
-// the published count.
+// Host side: pp_serve_core.hpp (admission by slots and hit-pool room,
+// publishing of range entries, reaping of completions, kernel stop and
+// relaunch), run here on a HIP backend.
+// Device side (persistent kernel, a fixed share of the resident block slots):
+// a block takes a ticket (the next global tile index) with one atomicAdd and
+// waits until the tile is published.  The published count is mirrored from
+// pinned memory into a device-memory gate word; only the block holding the
+// lowest unpublished ticket (the head) reads the host's counters over PCIe, so
+// the link carries one poller however many blocks wait.  Before it moves the
+// gate the head also copies the newly published range entries into a
+// device-memory ring, so a block finds its range entry by a 64-ary search in
+// device memory (a search over pinned memory, some 70 scattered reads a tile,
+// had held the serving kernel to 40 % of the batch kernel's throughput on the
+// same work).  The request's slot descriptor is read over PCIe only when the
+// tile belongs to another request than the block's previous tile (the entry
+// carries the request's tag); the block keeps it in LDS.  It runs the tile with
+// the same run_tile as the batch kernel, stores the tile partial, and the block
+// that completes a request's last tile adds the partials in tile order and
+// posts the summary to pinned memory.  The host stops the kernel (`stop`,
+// raised into the gate by the head) only when every published tile is done, so
+// the tickets held at exit are all unpublished and the next launch restarts the
+// ticket counter at the published count.  The head also stops the kernel by
+// itself after waiting wait_ticks with nothing published, reporting the tile
+// to resume at.
 
 namespace {
 
-constexpr uint32_t kSrvRing = 16384;          // range-entry ring
+using pps::kSrvRing;
+using pps::SrvControl;
+using pps::SrvDone;
+using pps::SrvEntry;
+using pps::SrvSlot;
+
 constexpr uint64_t kSrvExit = ~0ull;
 constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
 constexpr size_t kSrvGateOffset = 4096;  // ticket, gate and mirror counts on separate pages
 // Room left on every SIMD for other kernels: VGPRs per lane and wave slots (of
-// 8) the persistent grid does not take.  Without an SDMA engine the result
-// copies are runtime copy kernels that must find room beside it.
-constexpr int kSrvVgprReserve = 64;           // no SDMA engine
-constexpr int kSrvMaxWavesPerSimd = 4;
-constexpr int kSrvVgprReserveSdma = 32;       // result copies on an SDMA engine
-constexpr int kSrvMaxWavesPerSimdSdma = 5;
+// 8) the persistent grid does not take.  Result copies never need compute
+// units (an SDMA engine, or a hit pool in host memory), so this is for the
+// caller's other work on the device.  64 photons x 1e8 through the server took
+// 111.0, 101.4 and 99.2 ms at 5, 7 and 8 waves per SIMD (the batch kernel
+// 99.2 ms); 7 keeps one wave slot free on every SIMD.
+constexpr int kSrvVgprReserve = 32;
+constexpr int kSrvMaxWavesPerSimd = 7;
 constexpr uint64_t kSrvStageRecords = 1u << 16;  // 4 MiB per half of the result staging
-constexpr uint64_t kSrvChunkTiles = 64;     // smallest range entry of a large request
-constexpr uint64_t kSrvMaxChunks = 128;     // range entries per large request at most
-constexpr uint64_t kSrvSmallTiles = 512;    // requests up to this many tiles are small
-constexpr uint64_t kSrvBacklogGrids = 1;    // published, unclaimed tiles kept: grids' worth
 
-struct SrvEntry {       // global tiles [t0, t1) are the request's tiles local0..
-    uint64_t t0, t1;
-    uint32_t slot, pad;
-    uint64_t local0;
-};
-
-static_assert(sizeof(SrvEntry) % 8 == 0 && 64 % (sizeof(SrvEntry) / 8) == 0,
-              "a range entry is copied as whole 8-byte words, a whole number per wave");
-
-struct SrvSlot {       // written by the host before the tiles are published
-    DevReq req;
-    uint64_t n_tiles;
-    uint64_t hit_base;  // record offset into the hit pool
-    uint64_t hit_cap;
-    uint64_t seq;       // completion tag
-};
-
-static_assert(sizeof(SrvSlot) <= 64 * 4, "one wave loads the slot descriptor, a word a lane");
-
-struct SrvDone {       // written by the device
-    uint64_t words[8];  // pp_summary
-    uint64_t seq;
-    uint64_t pad[7];
-};
-
-struct SrvControl {
-    uint64_t published;   // end of the published global tile range
-    uint64_t n_entries;   // range entries published
-    uint64_t first_live;  // oldest entry that can still be searched
-    uint64_t stop;
-    uint64_t claimed;     // a recently claimed tile (every 64th; device-written)
-    uint64_t pad[3];
-};
-
-struct SrvParams {
+// The serving kernel's addresses and limits, in device memory (written once
+// per server).  The kernel reads each field where it uses it (SA below), so
+// that none of them occupies registers through the tile's step loop.
+struct SrvAddr {
     SrvControl* ctl;
     const SrvEntry* entries;
     const SrvSlot* slots;
@@ -1629,8 +1460,13 @@ struct SrvParams {
     uint32_t* tiles_done;             // [max_slots]
     unsigned long long* hit_cursor;   // [max_slots]
     TilePartial* partials;            // [max_slots][max_tiles]
-    pp_hit* pool;
-    uint32_t max_tiles;
+    pp_hit* pool;                     // device memory, or pinned host memory
+    uint64_t wait_ticks;              // s_memrealtime ticks the head waits before stopping
+    uint64_t max_tiles;
+};
+
+struct SrvParams {
+    const SrvAddr* addr;
     uint32_t n_points;
     const float4* table;
     const float* knots;
@@ -1638,6 +1474,17 @@ struct SrvParams {
     ppd::HG hg;
     SpecParams spec;
 };
+
+// Pointer p, opaque to the compiler at this point: a load through it is not
+// hoisted or merged with another, so the loaded value lives only from here.
+template <typename T>
+__device__ __forceinline__ const T* opaque_ptr(const T* p) {
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
+
+// A field of the serving addresses, read here with a scalar load.
+#define SA(field) load_constant(&opaque_ptr(P.addr)->field, 0)
 
 template <typename T>
 __device__ __forceinline__ T sys_load(const T* p) {
@@ -1661,6 +1508,19 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// A trivially copyable value that is the same in every lane (read from LDS),
+// moved word by word into scalar registers.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T r;
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(&v);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) d[k] = __builtin_amdgcn_readfirstlane(s[k]);
+    return r;
+}
+
 // Copy the host's range entries [mirrored, n_entries) into the device ring
 // (one wave; each lane moves one 8-byte word, 16 entries a pass, four passes
 // in flight).  Run by the head block before it moves the gate, so that every
@@ -1668,13 +1528,13 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
 // search there, not over PCIe.  The entries copied are those of unfinished
 // jobs: the host does not reuse their ring positions meanwhile.
 __device__ void mirror_entries(const SrvParams& P, uint32_t lane) {
-    const uint64_t ne = uniform_u64(sys_load(&P.ctl->n_entries));
-    const uint64_t fl = uniform_u64(sys_load(&P.ctl->first_live));
-    const uint64_t m = uniform_u64(agent_load(&P.dmeta[0]));
+    const uint64_t ne = uniform_u64(sys_load(&SA(ctl)->n_entries));
+    const uint64_t fl = uniform_u64(sys_load(&SA(ctl)->first_live));
+    const uint64_t m = uniform_u64(agent_load(&SA(dmeta)[0]));
     constexpr uint32_t kWords = sizeof(SrvEntry) / 8;
     constexpr uint32_t kPerPass = 64 / kWords;
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(P.entries);
-    uint64_t* dst = reinterpret_cast<uint64_t*>(P.dentries);
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(SA(entries));
+    uint64_t* dst = reinterpret_cast<uint64_t*>(SA(dentries));
     const uint32_t word = lane % kWords;
     for (uint64_t e0 = m; e0 < ne; e0 += 4 * kPerPass) {
         uint64_t v[4];
@@ -1692,8 +1552,8 @@ __device__ void mirror_entries(const SrvParams& P, uint32_t lane) {
     // the entries have landed before the count that admits them to searches
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) {
-        agent_store(&P.dmeta[1], (unsigned long long)fl);
-        agent_store(&P.dmeta[0], (unsigned long long)ne);
+        agent_store(&SA(dmeta)[1], (unsigned long long)fl);
+        agent_store(&SA(dmeta)[0], (unsigned long long)ne);
     }
     // the counts have landed before the gate moves
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1701,28 +1561,31 @@ __device__ void mirror_entries(const SrvParams& P, uint32_t lane) {
 
 // At most 96 VGPRs (5 waves per SIMD's worth): the grid takes 4 or 5 waves per
 // SIMD and (512 - reserve) / VGPRs must allow that; 113 would allow 3 or 4.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) serve_kernel(SrvParams P) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) serve_kernel(SrvParams P) {
     __shared__ unsigned long long s_tile;
     __shared__ SrvSlot s_desc;
     __shared__ uint64_t s_entry_t0, s_entry_local0;
     __shared__ uint32_t s_slot;
+    __shared__ uint32_t s_desc_slot, s_desc_tag;  // request whose descriptor s_desc holds
     __shared__ uint32_t s_last;
     __shared__ double s_red[kWaves][3];
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_desc_slot = ~0u;
 
     for (;;) {
         __syncthreads();  // the previous tile's shared words consumed
         if (tid < 64) {   // wave 0: ticket, gate, range entry, slot descriptor
             const uint32_t lane = tid;
             unsigned long long tk = 0;
-            if (lane == 0) tk = atomicAdd(P.ticket, 1ull);
+            if (lane == 0) tk = atomicAdd(SA(ticket), 1ull);
             const uint64_t t = uniform_u64(__shfl(tk, 0, 64));
             uint64_t got = kSrvExit;
+            uint64_t wait_start = 0;  // device clock when this block began to wait as head
             for (;;) {
-                const uint64_t g = uniform_u64(agent_load(P.gate));
+                const uint64_t g = uniform_u64(agent_load(SA(gate)));
                 if (t < (g & ~kGateStop)) {
                     got = t;
                     break;
@@ -1733,15 +1596,29 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
                     // Relaxed is enough: every later read of host-written data
                     // is itself an uncached system-scope load, issued after
                     // the branch on this one.
-                    const uint64_t hp = uniform_u64(sys_load(&P.ctl->published));
+                    const uint64_t hp = uniform_u64(sys_load(&SA(ctl)->published));
                     if (hp > t) {
                         mirror_entries(P, lane);
-                        if (lane == 0) atomicMax(P.gate, (unsigned long long)hp);
+                        if (lane == 0) atomicMax(SA(gate), (unsigned long long)hp);
                         got = t;
                         break;
                     }
-                    if (uniform_u64(sys_load(&P.ctl->stop))) {
-                        if (lane == 0) atomicOr(P.gate, kGateStop);
+                    if (uniform_u64(sys_load(&SA(ctl)->stop))) {
+                        if (lane == 0) atomicOr(SA(gate), kGateStop);
+                        break;
+                    }
+                    // nothing published for wait_ticks: stop, and report the
+                    // tile to resume at (every ticket below t was published
+                    // and is done or being run; none above it was published)
+                    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                    if (wait_start == 0) {
+                        wait_start = now;
+                    } else if (now - wait_start > SA(wait_ticks)) {
+                        if (lane == 0) {
+                            __hip_atomic_store(&SA(ctl)->self_stop, (uint64_t)(t + 1),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                            atomicOr(SA(gate), kGateStop);
+                        }
                         break;
                     }
                     __builtin_amdgcn_s_sleep(16);
@@ -1755,7 +1632,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
             if (lane == 0) {
                 // how far the claims have got, for the host's publishing pace
                 if (got != kSrvExit && (got & 63u) == 0)
-                    __hip_atomic_store(&P.ctl->claimed, got, __ATOMIC_RELAXED,
+                    __hip_atomic_store(&SA(ctl)->claimed, got, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
                 s_tile = got;
             }
@@ -1769,13 +1646,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
                 // right one; if no lane is ok, the counts are read again.
                 uint64_t lo = 0;
                 for (bool found = false; !found;) {
-                    lo = uniform_u64(agent_load(&P.dmeta[1]));
-                    uint64_t hi = uniform_u64(agent_load(&P.dmeta[0]));
+                    lo = uniform_u64(agent_load(&SA(dmeta)[1]));
+                    uint64_t hi = uniform_u64(agent_load(&SA(dmeta)[0]));
                     found = true;
                     while (hi - lo > 1) {
                         const uint64_t stride = (hi - lo + 63) / 64;
                         const uint64_t idx = lo + lane * stride;
-                        const bool ok = idx < hi && agent_load(&P.dentries[idx % kSrvRing].t0) <= t;
+                        const bool ok = idx < hi && agent_load(&SA(dentries)[idx % kSrvRing].t0) <= t;
                         const uint64_t bits = __ballot(ok);
                         if (bits == 0) {
                             found = false;
@@ -1786,35 +1663,49 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
                         hi = std::min(hi, lo + stride);
                     }
                 }
-                const SrvEntry* e = &P.dentries[lo % kSrvRing];
-                const uint32_t slot = (uint32_t)uniform_u64(agent_load(&e->slot));
+                const SrvEntry* e = &SA(dentries)[lo % kSrvRing];
+                const uint32_t slot = __builtin_amdgcn_readfirstlane(agent_load(&e->slot));
+                const uint32_t tag = __builtin_amdgcn_readfirstlane(agent_load(&e->tag));
                 if (lane == 0) {
                     s_entry_t0 = agent_load(&e->t0);
                     s_entry_local0 = agent_load(&e->local0);
                     s_slot = slot;
                 }
-                if (lane < sizeof(SrvSlot) / 4)
-                    reinterpret_cast<uint32_t*>(&s_desc)[lane] =
-                        sys_load(reinterpret_cast<const uint32_t*>(&P.slots[slot]) + lane);
+                // the slot descriptor over PCIe only for another request than
+                // the block's previous tile's (a slot's tag changes with the
+                // request it holds)
+                if (slot != __builtin_amdgcn_readfirstlane(s_desc_slot) ||
+                    tag != __builtin_amdgcn_readfirstlane(s_desc_tag)) {
+                    if (lane < sizeof(SrvSlot) / 4)
+                        reinterpret_cast<uint32_t*>(&s_desc)[lane] =
+                            sys_load(reinterpret_cast<const uint32_t*>(&SA(slots)[slot]) + lane);
+                    if (lane == 0) {
+                        s_desc_slot = slot;
+                        s_desc_tag = tag;
+                    }
+                }
             }
         }
         __syncthreads();
         const unsigned long long t = s_tile;
         if (t == kSrvExit) break;
-        const uint32_t slot = s_slot;
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(s_slot);
         const SrvSlot* sd = &s_desc;
-        const DevReq R = sd->req;
-        const uint64_t local = t - s_entry_t0 + s_entry_local0;
+        // the request's words from LDS into scalar registers, as the batch
+        // kernel holds them
+        const DevReq R = uniform_copy(sd->req);
+        const uint64_t local = uniform_u64(t - s_entry_t0 + s_entry_local0);
         const uint64_t first = local * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
-        const uint64_t n_tiles = sd->n_tiles;
-        const uint64_t hit_cap = sd->hit_cap;
-        const uint64_t seq = sd->seq;
-        TilePartial* parts = P.partials + (uint64_t)slot * P.max_tiles;
-        const TilePartial tp = run_tile_any<true>(T, R, slot, first, count,
-                                                  P.pool + sd->hit_base, hit_cap,
-                                                  &P.hit_cursor[slot], s_red, s_redi, s_hitbuf);
+        const uint64_t n_tiles = uniform_u64(sd->n_tiles);
+        const uint64_t hit_cap = uniform_u64(sd->hit_cap);
+        const uint64_t hit_base = uniform_u64(sd->hit_base);
+        const uint64_t seq = uniform_u64(sd->seq);
+        TilePartial* parts = SA(partials) + (uint64_t)slot * SA(max_tiles);
+        const TilePartial tp = run_tile_any<true>(T, R, slot, first, count, SA(pool) + hit_base,
+                                                  hit_cap, &SA(hit_cursor)[slot], s_red, s_redi,
+                                                  s_hitbuf);
         // every wave waits for its own (system-scope) hit stores to complete
         // before the barrier; a workgroup-scope fence would not wait for them
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1828,7 +1719,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
             for (int k = 0; k < 5; ++k)
                 __hip_atomic_store(q + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // stores done before the count
-            const uint32_t done = atomicAdd(&P.tiles_done[slot], 1u) + 1u;
+            const uint32_t done = atomicAdd(&SA(tiles_done)[slot], 1u) + 1u;
             s_last = done == n_tiles;
         }
         __syncthreads();
@@ -1836,18 +1727,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
             const pp_summary s = reduce_partials<true>(parts, n_tiles, R.n_photons);
             if (tid == 0) {
                 const unsigned long long cur = __hip_atomic_load(
-                    &P.hit_cursor[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    &SA(hit_cursor)[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 pp_summary o = s;
                 o.n_stored = cur < hit_cap ? cur : hit_cap;
                 o.status = o.n_hits > o.n_stored ? PP_ERR_OVERFLOW : PP_OK;
                 const uint64_t* w = reinterpret_cast<const uint64_t*>(&o);
                 for (int k = 0; k < 8; ++k)
-                    __hip_atomic_store(&P.done[slot].words[k], w[k], __ATOMIC_RELAXED,
+                    __hip_atomic_store(&SA(done)[slot].words[k], w[k], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
-                atomicExch(&P.tiles_done[slot], 0u);
-                atomicExch(&P.hit_cursor[slot], 0ull);
+                atomicExch(&SA(tiles_done)[slot], 0u);
+                atomicExch(&SA(hit_cursor)[slot], 0ull);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // summary before its tag
-                __hip_atomic_store(&P.done[slot].seq, seq, __ATOMIC_RELAXED,
+                __hip_atomic_store(&SA(done)[slot].seq, seq, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
@@ -1863,358 +1754,9 @@ __global__ void serve_reset_kernel(unsigned long long* ticket, unsigned long lon
     }
 }
 
-struct Job {
-    uint64_t ticket = 0;
-    pp_request req;
-    uint64_t hits_cap = 0;
-    uint64_t n_tiles = 0;
-    uint32_t slot = 0;
-    uint64_t hit_base = 0;
-    uint64_t seq = 0;
-    uint64_t next_local = 0;  // tiles published so far
-    bool done = false;
-    pp_summary summary;
-    std::chrono::steady_clock::time_point t_submit, t_done;
-};
-
 }  // namespace
 
-struct pp_server {
-    pp_ctx* ctx = nullptr;
-    pp_server_config cfg;
-    SpecParams spec;
-    int grid = 0;
-    uint32_t n_points = 0;
-    float lmin = 0.0f, inv_dl = 0.0f, g = 0.0f;
-    float n_min = 0.0f;          // smallest n of the table snapshot (Cherenkov check)
-    hipStream_t kstream = nullptr, cstream = nullptr;
-    hipEvent_t kdone = nullptr;
-    bool kernel_live = false;
-    bool stopping = false;       // stop raised, kernel not yet finished: admit nothing
-    uint64_t resume_tile = 0;    // tiles processed when the kernel last stopped
-    std::chrono::steady_clock::time_point idle_since;
-    uint64_t launches = 0;
-    // pinned
-    SrvControl* ctl = nullptr;
-    SrvEntry* entries = nullptr;
-    SrvSlot* slots = nullptr;
-    SrvDone* done = nullptr;
-    // device
-    DevBuf table, knots, dctl, dentries, tiles_done, hit_cursor, partials, pool;
-    // result copies on an SDMA engine (no compute units): two pinned staging
-    // halves of kSrvStageRecords records, one completion signal each
-    bool hsa_up = false, sdma = false;
-    hsa_agent_t gpu_agent{}, cpu_agent{};
-    hsa_amd_sdma_engine_id_t engine{};
-    hsa_signal_t csig[2] = {};
-    pp_hit* stage = nullptr;
-    std::mutex copy_mu;
-    // host bookkeeping (guarded by mu)
-    std::mutex mu;
-    std::condition_variable cv;
-    std::thread worker;
-    bool shutdown = false;
-    std::deque<uint64_t> pending;
-    std::deque<uint64_t> small_q;             // admitted small jobs not yet published
-    std::deque<uint64_t> large_q;             // admitted large jobs with tiles left (FIFO)
-    uint64_t claimed_seen = 0;                // highest claimed tile seen
-    std::chrono::steady_clock::time_point claim_change;
-    std::map<uint64_t, Job> jobs;
-    std::vector<uint32_t> free_slots;
-    std::vector<uint64_t> slot_ticket;
-    std::vector<uint64_t> entry_ticket;       // ring: ticket of each live entry
-    std::map<uint64_t, uint64_t> pool_free;   // offset -> length (records)
-    uint64_t pool_free_total = 0;
-    uint64_t next_ticket = 1, next_seq = 1;
-    uint64_t published = 0, n_entries = 0, first_live = 0;
-    uint64_t submitted = 0, completed = 0;
-    std::string worker_error;
-};
-
 namespace {
-
-bool pool_alloc(pp_server* s, uint64_t n, uint64_t* off) {
-    if (n == 0) {
-        *off = 0;
-        return true;
-    }
-    for (auto it = s->pool_free.begin(); it != s->pool_free.end(); ++it) {
-        if (it->second >= n) {
-            *off = it->first;
-            const uint64_t rest = it->second - n, at = it->first + n;
-            s->pool_free.erase(it);
-            if (rest) s->pool_free[at] = rest;
-            s->pool_free_total -= n;
-            return true;
-        }
-    }
-    return false;
-}
-
-void pool_release(pp_server* s, uint64_t off, uint64_t n) {
-    if (n == 0) return;
-    s->pool_free_total += n;
-    auto it = s->pool_free.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != s->pool_free.end() && it->first + it->second == nx->first) {
-        it->second += nx->second;
-        s->pool_free.erase(nx);
-    }
-    if (it != s->pool_free.begin()) {
-        auto pv = std::prev(it);
-        if (pv->first + pv->second == it->first) {
-            pv->second += it->second;
-            s->pool_free.erase(it);
-        }
-    }
-}
-
-int launch_serve_kernel(pp_server* s) {
-    SrvParams P;
-    memset(&P, 0, sizeof P);
-    P.ctl = s->ctl;
-    P.entries = s->entries;
-    P.slots = s->slots;
-    P.done = s->done;
-    P.ticket = (unsigned long long*)s->dctl.p;
-    P.gate = (unsigned long long*)((char*)s->dctl.p + kSrvGateOffset);
-    P.dmeta = (unsigned long long*)((char*)s->dctl.p + 2 * kSrvGateOffset);
-    P.dentries = (SrvEntry*)s->dentries.p;
-    P.tiles_done = (uint32_t*)s->tiles_done.p;
-    P.hit_cursor = (unsigned long long*)s->hit_cursor.p;
-    P.partials = (TilePartial*)s->partials.p;
-    P.pool = (pp_hit*)s->pool.p;
-    P.max_tiles = s->cfg.max_tiles_per_request;
-    P.n_points = s->n_points;
-    P.table = (const float4*)s->table.p;
-    P.knots = (const float*)s->knots.p;
-    P.lmin = s->lmin;
-    P.inv_dl = s->inv_dl;
-    P.hg = make_hg(s->g);
-    P.spec = s->spec;
-    // every tile below resume_tile is done and none above it was started
-    serve_reset_kernel<<<1, 64, 0, s->kstream>>>(P.ticket, P.gate, s->resume_tile);
-    serve_kernel<<<s->grid, kBlock, 0, s->kstream>>>(P);
-    PP_HIP(hipGetLastError());
-    PP_HIP(hipEventRecord(s->kdone, s->kstream));
-    s->kernel_live = true;
-    ++s->launches;
-    return PP_OK;
-}
-
-// Append one range entry: tiles local0 .. local0 + c of job j.
-void publish_entry(pp_server* s, Job& j, uint64_t c) {
-    const uint64_t e = s->n_entries;
-    SrvEntry en;
-    memset(&en, 0, sizeof en);
-    en.t0 = s->published;
-    en.t1 = s->published + c;
-    en.slot = j.slot;
-    en.local0 = j.next_local;
-    memcpy(&s->entries[e % kSrvRing], &en, sizeof en);
-    s->entry_ticket[e % kSrvRing] = j.ticket;
-    s->published = en.t1;
-    s->n_entries = e + 1;
-    j.next_local += c;
-}
-
-// Publish admitted jobs' tiles.  Small jobs (up to kSrvSmallTiles tiles) go
-// out whole as soon as they are admitted, ahead of everything else.  Large
-// jobs go out first come first served, in chunks, while fewer than
-// kSrvBacklogGrids grids' worth of published tiles wait to be claimed.  So a small request waits for
-// at most that backlog, not for the tiles of the large requests before it,
-// and large requests keep their arrival order.  Each request's result does not
-// depend on the interleaving.
-//
-// Range entries retire in order (first_live stops at the oldest unfinished
-// job).  Once the ring is half full only that job may publish: it is either
-// fully published or the large job being published, needs at most
-// kSrvMaxChunks entries, and so always finishes, which frees the ring.
-void publish_locked(pp_server* s) {
-    if (s->stopping) return;  // tickets restart at resume_tile on the next launch
-    // Claim reports are plain stores from many CUs and may land out of order;
-    // the highest seen is kept.  Should the estimate stay full with no new
-    // report for 1 ms (an older report landing last), everything published is
-    // taken as claimed: that can only publish early, never stall the kernel.
-    const auto now = std::chrono::steady_clock::now();
-    const uint64_t target = std::max<uint64_t>(kSrvBacklogGrids * (uint64_t)s->grid, 4 * kSrvChunkTiles);
-    const uint64_t seen = __atomic_load_n(&s->ctl->claimed, __ATOMIC_ACQUIRE);
-    if (seen > s->claimed_seen && seen <= s->published) {
-        s->claimed_seen = seen;
-        s->claim_change = now;
-    } else if (s->published - s->claimed_seen >= target &&
-               now - s->claim_change > std::chrono::milliseconds(1)) {
-        s->claimed_seen = s->published;
-        s->claim_change = now;
-    }
-    const uint64_t n_before = s->n_entries;
-    auto in_use = [s] { return s->n_entries - s->first_live; };
-    const uint64_t pin = s->first_live < s->n_entries ? s->entry_ticket[s->first_live % kSrvRing] : 0;
-    while (!s->small_q.empty() && in_use() < kSrvRing / 2) {
-        Job& j = s->jobs[s->small_q.front()];
-        s->small_q.pop_front();
-        publish_entry(s, j, j.n_tiles);
-    }
-    while (!s->large_q.empty() && s->published - s->claimed_seen < target) {
-        Job& j = s->jobs[s->large_q.front()];
-        if (in_use() >= kSrvRing / 2 && j.ticket != pin && in_use() > 0) break;
-        if (in_use() >= kSrvRing) break;
-        const uint64_t chunk = std::max(kSrvChunkTiles, (j.n_tiles + kSrvMaxChunks - 1) / kSrvMaxChunks);
-        publish_entry(s, j, std::min(chunk, j.n_tiles - j.next_local));
-        if (j.next_local == j.n_tiles) s->large_q.pop_front();
-    }
-    if (s->n_entries != n_before) {
-        std::atomic_thread_fence(std::memory_order_seq_cst);
-        __atomic_store_n(&s->ctl->n_entries, s->n_entries, __ATOMIC_RELEASE);
-        __atomic_store_n(&s->ctl->published, s->published, __ATOMIC_RELEASE);
-    }
-}
-
-// Admit pending jobs (FIFO) while a slot and pool room exist, then publish.
-// A job without photons completes at once and takes neither a slot nor pool
-// records (its hits_cap reservation is zero), so no slot ever carries the
-// ticket of a job that the device will not complete.
-void admit_locked(pp_server* s) {
-    if (s->stopping) return;  // tickets restart at resume_tile on the next launch
-    while (!s->pending.empty()) {
-        auto it = s->jobs.find(s->pending.front());
-        if (it == s->jobs.end()) {  // cannot happen: pending jobs are never erased
-            s->pending.pop_front();
-            continue;
-        }
-        Job& j = it->second;
-        if (j.n_tiles == 0) {
-            s->pending.pop_front();
-            memset(&j.summary, 0, sizeof j.summary);
-            j.done = true;
-            j.t_done = std::chrono::steady_clock::now();
-            ++s->completed;
-            continue;
-        }
-        if (s->free_slots.empty()) break;
-        uint64_t off = 0;
-        if (!pool_alloc(s, j.hits_cap, &off)) break;
-        s->pending.pop_front();
-        j.slot = s->free_slots.back();
-        s->free_slots.pop_back();
-        j.hit_base = off;
-        j.seq = s->next_seq++;
-        s->slot_ticket[j.slot] = j.ticket;
-        SrvSlot sd;
-        memset(&sd, 0, sizeof sd);
-        make_devreq(&j.req, j.slot, &sd.req);
-        sd.n_tiles = j.n_tiles;
-        sd.hit_base = off;
-        sd.hit_cap = j.hits_cap;
-        sd.seq = j.seq;
-        memcpy(&s->slots[j.slot], &sd, sizeof sd);
-        j.next_local = 0;
-        (j.n_tiles <= kSrvSmallTiles ? s->small_q : s->large_q).push_back(j.ticket);
-    }
-    publish_locked(s);
-}
-
-// Completions posted by the device; advance first_live past finished entries.
-bool reap_locked(pp_server* s) {
-    bool any = false;
-    for (uint32_t slot = 0; slot < s->cfg.max_slots; ++slot) {
-        const uint64_t tk = s->slot_ticket[slot];
-        if (!tk) continue;
-        // a slot's ticket is cleared when its job completes, before the job
-        // can be collected and erased; never insert a job here
-        auto it = s->jobs.find(tk);
-        if (it == s->jobs.end()) continue;
-        Job& j = it->second;
-        if (j.done) continue;
-        if (__atomic_load_n(&s->done[slot].seq, __ATOMIC_ACQUIRE) != j.seq) continue;
-        memcpy(&j.summary, s->done[slot].words, sizeof(pp_summary));
-        j.done = true;
-        j.t_done = std::chrono::steady_clock::now();
-        s->slot_ticket[slot] = 0;
-        s->free_slots.push_back(slot);
-        ++s->completed;
-        any = true;
-    }
-    while (s->first_live < s->n_entries) {
-        const uint64_t tk = s->entry_ticket[s->first_live % kSrvRing];
-        auto it = s->jobs.find(tk);
-        if (it != s->jobs.end() && !it->second.done) break;
-        ++s->first_live;
-    }
-    if (any) __atomic_store_n(&s->ctl->first_live, s->first_live, __ATOMIC_RELEASE);
-    return any;
-}
-
-void worker_main(pp_server* s) {
-    if (hipSetDevice(s->ctx->device) != hipSuccess) {
-        std::lock_guard<std::mutex> g(s->mu);
-        s->worker_error = "hipSetDevice failed in the serving worker";
-        return;
-    }
-    std::unique_lock<std::mutex> lk(s->mu);
-    while (!s->shutdown) {
-        const bool any = reap_locked(s);
-        admit_locked(s);
-        if (any) s->cv.notify_all();
-        if (s->kernel_live && hipEventQuery(s->kdone) == hipSuccess) {
-            s->kernel_live = false;
-            if (s->stopping) {
-                s->stopping = false;
-                __atomic_store_n(&s->ctl->stop, 0ull, __ATOMIC_RELEASE);
-                admit_locked(s);
-            }
-        }
-        // Busy while admitted requests are in flight.  Requests that wait for
-        // a slot or pool room do not count: only collecting results frees
-        // those, and a caller blocked on another kernel could not collect
-        // while this kernel held the compute units.
-        const bool busy = s->submitted - s->completed > s->pending.size();
-        const auto now = std::chrono::steady_clock::now();
-        if (busy) s->idle_since = now;
-        if (s->kernel_live && !s->stopping && !busy &&
-            now - s->idle_since > std::chrono::microseconds(s->cfg.idle_exit_us)) {
-            // every published tile is done: the blocks all hold unpublished
-            // tickets, and the next launch resumes from here
-            s->stopping = true;
-            s->resume_tile = s->published;
-            s->claimed_seen = s->published;
-            __atomic_store_n(&s->ctl->stop, 1ull, __ATOMIC_RELEASE);
-        }
-        if (busy && !s->kernel_live && s->published > s->resume_tile) {
-            if (launch_serve_kernel(s) != PP_OK) {
-                s->worker_error = last_error();
-                s->cv.notify_all();
-                break;
-            }
-        }
-        if (busy) {
-            lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
-            lk.lock();
-        } else {
-            s->cv.wait_for(lk, std::chrono::milliseconds(5));
-        }
-    }
-}
-
-void free_server(pp_server* s) {
-    if (s->ctl) (void)hipHostFree(s->ctl);
-    if (s->entries) (void)hipHostFree(s->entries);
-    if (s->slots) (void)hipHostFree(s->slots);
-    if (s->done) (void)hipHostFree(s->done);
-    DevBuf* bufs[] = {&s->table, &s->knots, &s->dctl, &s->dentries, &s->tiles_done, &s->hit_cursor,
-                      &s->partials, &s->pool};
-    for (DevBuf* b : bufs)
-        if (b->p) (void)hipFree(b->p);
-    if (s->kdone) (void)hipEventDestroy(s->kdone);
-    if (s->kstream) (void)hipStreamDestroy(s->kstream);
-    if (s->cstream) (void)hipStreamDestroy(s->cstream);
-    for (hsa_signal_t& sg : s->csig)
-        if (sg.handle) (void)hsa_signal_destroy(sg);
-    if (s->stage) (void)hipHostFree(s->stage);
-    if (s->hsa_up) (void)hsa_shut_down();
-    delete s;
-}
 
 hsa_status_t find_cpu_agent(hsa_agent_t a, void* data) {
     hsa_device_type_t t;
@@ -2226,134 +1768,272 @@ hsa_status_t find_cpu_agent(hsa_agent_t a, void* data) {
     return HSA_STATUS_SUCCESS;
 }
 
-// Start an SDMA copy of n records from the pool to staging half h.
-bool sdma_issue(pp_server* s, int h, const pp_hit* src, uint64_t n) {
-    hsa_signal_store_screlease(s->csig[h], 1);
-    return hsa_amd_memory_async_copy_on_engine(s->stage + (uint64_t)h * kSrvStageRecords,
-                                               s->cpu_agent, src, s->gpu_agent,
-                                               n * sizeof(pp_hit), 0, nullptr, s->csig[h],
-                                               s->engine, false) == HSA_STATUS_SUCCESS;
-}
+// The serving loop's device side on HIP: the persistent kernel, its buffers,
+// the pinned memory it shares with the host, and the result copies.  Result
+// records never need compute units to leave the device: they go from a device
+// pool through an SDMA engine, or the pool is pinned host memory (flag
+// PP_SERVER_HOST_POOL, or no SDMA engine), which the kernel writes directly.
+struct HipBackend {
+    pp_ctx* ctx = nullptr;
+    pp_server_config cfg{};
+    SpecParams spec{};
+    uint32_t n_points = 0;
+    float lmin = 0.0f, inv_dl = 0.0f, g = 0.0f;
+    int grid_ = 0;
+    uint64_t wait_ticks = 0;
+    hipStream_t kstream = nullptr;
+    hipEvent_t kdone = nullptr;
+    bool launched = false;
+    // pinned, shared with the kernel
+    SrvControl* ctl = nullptr;
+    SrvEntry* entries = nullptr;
+    SrvSlot* slots = nullptr;
+    SrvDone* done = nullptr;
+    // device
+    DevBuf table, knots, dctl, dentries, tiles_done, hit_cursor, partials, pool, addr;
+    pp_hit* host_pool = nullptr;  // the hit pool in pinned host memory
+    // result copies on an SDMA engine (no compute units): two pinned staging
+    // halves of kSrvStageRecords records, one completion signal each
+    bool hsa_up = false, sdma = false;
+    hsa_agent_t gpu_agent{}, cpu_agent{};
+    hsa_amd_sdma_engine_id_t engine{};
+    hsa_signal_t csig[2] = {};
+    pp_hit* stage = nullptr;
+    std::mutex copy_mu;
 
-// Wait for staging half h; false on a copy error or after timeout_ms.
-bool sdma_wait(pp_server* s, int h, int64_t timeout_ms) {
-    const auto t_end = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
-    for (;;) {
-        const hsa_signal_value_t v = hsa_signal_wait_scacquire(
-            s->csig[h], HSA_SIGNAL_CONDITION_LT, 1, 1000000, HSA_WAIT_STATE_ACTIVE);
-        if (v < 0) return false;
-        if (v == 0) return true;
-        if (std::chrono::steady_clock::now() > t_end) return false;
+    ~HipBackend() {
+        if (ctx) (void)hipSetDevice(ctx->device);
+        if (kstream) (void)hipStreamSynchronize(kstream);
+        if (ctl) (void)hipHostFree(ctl);
+        if (entries) (void)hipHostFree(entries);
+        if (slots) (void)hipHostFree(slots);
+        if (done) (void)hipHostFree(done);
+        if (host_pool) (void)hipHostFree(host_pool);
+        DevBuf* bufs[] = {&table, &knots, &dctl, &dentries, &tiles_done, &hit_cursor, &partials, &pool,
+                          &addr};
+        for (DevBuf* b : bufs)
+            if (b->p) (void)hipFree(b->p);
+        if (kdone) (void)hipEventDestroy(kdone);
+        if (kstream) (void)hipStreamDestroy(kstream);
+        for (hsa_signal_t& sg : csig)
+            if (sg.handle) (void)hsa_signal_destroy(sg);
+        if (stage) (void)hipHostFree(stage);
+        if (hsa_up) (void)hsa_shut_down();
     }
-}
 
-// Set up result copies on an SDMA engine: the HSA agents of the pool's GPU and
-// of host memory, an engine with a path between them, pinned staging and one
-// test copy.  Returns false (and leaves s->sdma false) if any step fails; the
-// server then copies with hipMemcpyAsync.
-bool setup_sdma(pp_server* s) {
-    if (hsa_init() != HSA_STATUS_SUCCESS) return false;
-    s->hsa_up = true;
-    hsa_amd_pointer_info_t pi;
-    memset(&pi, 0, sizeof pi);
-    pi.size = sizeof pi;
-    if (hsa_amd_pointer_info(s->pool.p, &pi, nullptr, nullptr, nullptr) != HSA_STATUS_SUCCESS ||
-        pi.type != HSA_EXT_POINTER_TYPE_HSA)
-        return false;
-    s->gpu_agent = pi.agentOwner;
-    if (hsa_iterate_agents(find_cpu_agent, &s->cpu_agent) != HSA_STATUS_INFO_BREAK) return false;
-    uint32_t mask = 0;
-    if (hsa_amd_memory_copy_engine_status(s->cpu_agent, s->gpu_agent, &mask) != HSA_STATUS_SUCCESS ||
-        mask == 0)
-        return false;
-    s->engine = (hsa_amd_sdma_engine_id_t)(mask & (~mask + 1));  // lowest available
-    for (hsa_signal_t& sg : s->csig)
-        if (hsa_signal_create(0, 0, nullptr, &sg) != HSA_STATUS_SUCCESS) return false;
-    if (hipHostMalloc((void**)&s->stage, 2 * kSrvStageRecords * sizeof(pp_hit), 0) != hipSuccess) {
-        s->stage = nullptr;
-        return false;
+    pps::SrvShared shared() { return {ctl, entries, slots, done}; }
+
+    int thread_init() {
+        PP_HIP(hipSetDevice(ctx->device));
+        return PP_OK;
     }
-    // one record through the engine and back to the host
-    if (!sdma_issue(s, 0, (const pp_hit*)s->pool.p, 1) || !sdma_wait(s, 0, 2000)) return false;
-    s->sdma = true;
-    return true;
-}
 
-int server_setup(pp_server* s, const pp_spectrum* spectrum) {
-    pp_ctx* c = s->ctx;
-    int rc;
-    PP_HIP(hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking));
-    PP_HIP(hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking));
-    PP_HIP(hipEventCreateWithFlags(&s->kdone, hipEventDisableTiming));
-    if ((rc = check_spectrum(spectrum, &s->spec))) return rc;
-    if ((rc = ensure(s->knots, PP_MAX_KNOTS * sizeof(float)))) return rc;
-    if ((rc = upload_knots(spectrum, (float*)s->knots.p, c->stream))) return rc;
-    // the server keeps its own copy of the medium table (pp_set_medium on the
-    // context does not change requests already being served)
-    if ((rc = ensure(s->table, (size_t)c->n_points * sizeof(float4)))) return rc;
-    PP_HIP(hipMemcpyAsync(s->table.p, c->table.p, (size_t)c->n_points * sizeof(float4),
-                          hipMemcpyDeviceToDevice, c->stream));
-    s->n_points = c->n_points;
-    s->lmin = c->lmin;
-    s->inv_dl = c->inv_dl;
-    s->g = c->g;
-    s->n_min = c->n_min;
-    PP_HIP(hipStreamSynchronize(c->stream));
-    const uint32_t S = s->cfg.max_slots;
-    PP_HIP(hipHostMalloc((void**)&s->ctl, sizeof(SrvControl), hipHostMallocCoherent));
-    PP_HIP(hipHostMalloc((void**)&s->entries, kSrvRing * sizeof(SrvEntry), hipHostMallocCoherent));
-    PP_HIP(hipHostMalloc((void**)&s->slots, S * sizeof(SrvSlot), hipHostMallocCoherent));
-    PP_HIP(hipHostMalloc((void**)&s->done, S * sizeof(SrvDone), hipHostMallocCoherent));
-    memset(s->ctl, 0, sizeof(SrvControl));
-    memset(s->entries, 0, kSrvRing * sizeof(SrvEntry));
-    memset(s->slots, 0, S * sizeof(SrvSlot));
-    memset(s->done, 0, S * sizeof(SrvDone));
-    if ((rc = ensure(s->dctl, 3 * kSrvGateOffset))) return rc;
-    if ((rc = ensure(s->dentries, kSrvRing * sizeof(SrvEntry)))) return rc;
-    if ((rc = ensure(s->tiles_done, S * sizeof(uint32_t)))) return rc;
-    if ((rc = ensure(s->hit_cursor, S * sizeof(unsigned long long)))) return rc;
-    if ((rc = ensure(s->partials, (size_t)S * s->cfg.max_tiles_per_request * sizeof(TilePartial))))
-        return rc;
-    if ((rc = ensure(s->pool, std::max<uint64_t>(s->cfg.hit_pool_records, 1) * sizeof(pp_hit))))
-        return rc;
-    fill_u32_kernel<<<8, 256, 0, s->kstream>>>((uint32_t*)s->dctl.p, 3 * kSrvGateOffset / 4, 0u);
-    fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->tiles_done.p, S, 0u);
-    fill_u32_kernel<<<64, 256, 0, s->kstream>>>((uint32_t*)s->hit_cursor.p, 2ull * S, 0u);
-    PP_HIP(hipGetLastError());
-    PP_HIP(hipStreamSynchronize(s->kstream));
-    // Result records leave the device pool on an SDMA engine where one is
-    // available (no compute units involved), else through hipMemcpyAsync,
-    // whose runtime copy kernel needs room on a CU: had the persistent kernel
-    // filled them, a result copy would wait for the kernel to exit while the
-    // kernel waits (busy) for the request that the copy's pool release would
-    // admit (7 waves per SIMD, leaving 8 or 64 VGPRs and one wave slot,
-    // deadlocked this way).  Either way the grid leaves room on every CU for
-    // other kernels: each block puts one wave on each SIMD; without an SDMA
-    // engine 64 of the 512 VGPRs per lane and 4 of the 8 wave slots stay free,
-    // with one 32 VGPRs and 3 slots (5 waves per SIMD ran the same 64 x 1e8
-    // photons 7 % faster than 4: 113 against 121 ms).  (A 64-VGPR build
-    // filling all 8 slots served the mixed request load 20-25 % slower than one
-    // at 4: its spills cost more than the extra waves.)
-    setup_sdma(s);
-    int per_cu = 0;
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, 0));
-    hipFuncAttributes fa;
-    PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
-    const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
-    const int reserve = s->sdma ? kSrvVgprReserveSdma : kSrvVgprReserve;
-    const int max_waves = s->sdma ? kSrvMaxWavesPerSimdSdma : kSrvMaxWavesPerSimd;
-    per_cu = std::min({per_cu, (512 - reserve) / vgpr, max_waves});
-    s->grid = std::max(1, per_cu) * c->n_cu;
-    s->free_slots.reserve(S);
-    for (uint32_t i = 0; i < S; ++i) s->free_slots.push_back(S - 1 - i);
-    s->slot_ticket.assign(S, 0);
-    s->entry_ticket.assign(kSrvRing, 0);
-    s->pool_free_total = s->cfg.hit_pool_records;
-    if (s->cfg.hit_pool_records) s->pool_free[0] = s->cfg.hit_pool_records;
-    return PP_OK;
-}
+    uint64_t grid() const { return (uint64_t)grid_; }
+
+    bool running() { return launched && hipEventQuery(kdone) == hipErrorNotReady; }
+
+    void finish() {
+        if (kstream) (void)hipStreamSynchronize(kstream);
+    }
+
+    int launch(uint64_t resume_tile) {
+        SrvParams P;
+        memset(&P, 0, sizeof P);
+        P.addr = (const SrvAddr*)addr.p;
+        P.n_points = n_points;
+        P.table = (const float4*)table.p;
+        P.knots = (const float*)knots.p;
+        P.lmin = lmin;
+        P.inv_dl = inv_dl;
+        P.hg = make_hg(g);
+        P.spec = spec;
+        // every tile below resume_tile is done and none above it was started
+        serve_reset_kernel<<<1, 64, 0, kstream>>>(
+            (unsigned long long*)dctl.p, (unsigned long long*)((char*)dctl.p + kSrvGateOffset),
+            resume_tile);
+        serve_kernel<<<grid_, kBlock, 0, kstream>>>(P);
+        PP_HIP(hipGetLastError());
+        PP_HIP(hipEventRecord(kdone, kstream));
+        launched = true;
+        return PP_OK;
+    }
+
+    // Start an SDMA copy of n records from the pool to staging half h.
+    bool sdma_issue(int h, const pp_hit* src, uint64_t n) {
+        hsa_signal_store_screlease(csig[h], 1);
+        return hsa_amd_memory_async_copy_on_engine(stage + (uint64_t)h * kSrvStageRecords, cpu_agent,
+                                                   src, gpu_agent, n * sizeof(pp_hit), 0, nullptr,
+                                                   csig[h], engine, false) == HSA_STATUS_SUCCESS;
+    }
+
+    // Wait for staging half h; false on a copy error or after timeout_ms.
+    bool sdma_wait(int h, int64_t timeout_ms) {
+        const auto t_end = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
+        for (;;) {
+            const hsa_signal_value_t v = hsa_signal_wait_scacquire(
+                csig[h], HSA_SIGNAL_CONDITION_LT, 1, 1000000, HSA_WAIT_STATE_ACTIVE);
+            if (v < 0) return false;
+            if (v == 0) return true;
+            if (std::chrono::steady_clock::now() > t_end) return false;
+        }
+    }
+
+    // Set up result copies on an SDMA engine: the HSA agents of the pool's GPU
+    // and of host memory, an engine with a path between them, pinned staging
+    // and one test copy.  Returns false (sdma stays false) if any step fails.
+    bool setup_sdma() {
+        if (hsa_init() != HSA_STATUS_SUCCESS) return false;
+        hsa_up = true;
+        hsa_amd_pointer_info_t pi;
+        memset(&pi, 0, sizeof pi);
+        pi.size = sizeof pi;
+        if (hsa_amd_pointer_info(pool.p, &pi, nullptr, nullptr, nullptr) != HSA_STATUS_SUCCESS ||
+            pi.type != HSA_EXT_POINTER_TYPE_HSA)
+            return false;
+        gpu_agent = pi.agentOwner;
+        if (hsa_iterate_agents(find_cpu_agent, &cpu_agent) != HSA_STATUS_INFO_BREAK) return false;
+        uint32_t mask = 0;
+        if (hsa_amd_memory_copy_engine_status(cpu_agent, gpu_agent, &mask) != HSA_STATUS_SUCCESS ||
+            mask == 0)
+            return false;
+        engine = (hsa_amd_sdma_engine_id_t)(mask & (~mask + 1));  // lowest available
+        for (hsa_signal_t& sg : csig)
+            if (hsa_signal_create(0, 0, nullptr, &sg) != HSA_STATUS_SUCCESS) return false;
+        if (hipHostMalloc((void**)&stage, 2 * kSrvStageRecords * sizeof(pp_hit), 0) != hipSuccess) {
+            stage = nullptr;
+            return false;
+        }
+        // one record through the engine and back to the host
+        if (!sdma_issue(0, (const pp_hit*)pool.p, 1) || !sdma_wait(0, 2000)) return false;
+        sdma = true;
+        return true;
+    }
+
+    int copy_hits(uint64_t base, uint64_t n, pp_hit* out) {
+        if (host_pool) {  // the kernel wrote them to host memory; seen through the completion tag
+            parallel_memcpy(reinterpret_cast<char*>(out), reinterpret_cast<const char*>(host_pool + base),
+                            n * sizeof(pp_hit));
+            return PP_OK;
+        }
+        // pool -> pinned staging on the SDMA engine, staging -> caller on the
+        // host, the two halves alternating so that both overlap
+        std::lock_guard<std::mutex> cg(copy_mu);
+        const pp_hit* src = (const pp_hit*)pool.p + base;
+        uint64_t off = 0;
+        int h = 0;
+        if (!sdma_issue(0, src, std::min(n, kSrvStageRecords)))
+            return fail(PP_ERR_DEVICE, "SDMA copy of result records failed to start");
+        while (off < n) {
+            const uint64_t m = std::min(n - off, kSrvStageRecords);
+            if (!sdma_wait(h, 60000)) return fail(PP_ERR_DEVICE, "SDMA copy of result records failed");
+            const uint64_t next = off + m;
+            if (next < n && !sdma_issue(h ^ 1, src + next, std::min(n - next, kSrvStageRecords)))
+                return fail(PP_ERR_DEVICE, "SDMA copy of result records failed to start");
+            parallel_memcpy(reinterpret_cast<char*>(out + off),
+                            reinterpret_cast<const char*>(stage + (uint64_t)h * kSrvStageRecords),
+                            m * sizeof(pp_hit));
+            off = next;
+            h ^= 1;
+        }
+        return PP_OK;
+    }
+
+    int setup(pp_ctx* c, const pp_spectrum* spectrum, const pp_server_config& config) {
+        ctx = c;
+        cfg = config;
+        int rc;
+        PP_HIP(hipStreamCreateWithFlags(&kstream, hipStreamNonBlocking));
+        PP_HIP(hipEventCreateWithFlags(&kdone, hipEventDisableTiming));
+        if ((rc = check_spectrum(spectrum, &spec))) return rc;
+        if ((rc = ensure(knots, PP_MAX_KNOTS * sizeof(float)))) return rc;
+        if ((rc = upload_knots(spectrum, (float*)knots.p, c->stream))) return rc;
+        // the server keeps its own copy of the medium table (pp_set_medium on
+        // the context does not change requests already being served)
+        if ((rc = ensure(table, (size_t)c->n_points * sizeof(float4)))) return rc;
+        PP_HIP(hipMemcpyAsync(table.p, c->table.p, (size_t)c->n_points * sizeof(float4),
+                              hipMemcpyDeviceToDevice, c->stream));
+        n_points = c->n_points;
+        lmin = c->lmin;
+        inv_dl = c->inv_dl;
+        g = c->g;
+        PP_HIP(hipStreamSynchronize(c->stream));
+        const uint32_t S = cfg.max_slots;
+        PP_HIP(hipHostMalloc((void**)&ctl, sizeof(SrvControl), hipHostMallocCoherent));
+        PP_HIP(hipHostMalloc((void**)&entries, kSrvRing * sizeof(SrvEntry), hipHostMallocCoherent));
+        PP_HIP(hipHostMalloc((void**)&slots, S * sizeof(SrvSlot), hipHostMallocCoherent));
+        PP_HIP(hipHostMalloc((void**)&done, S * sizeof(SrvDone), hipHostMallocCoherent));
+        memset(ctl, 0, sizeof(SrvControl));
+        memset(entries, 0, kSrvRing * sizeof(SrvEntry));
+        memset(slots, 0, S * sizeof(SrvSlot));
+        memset(done, 0, S * sizeof(SrvDone));
+        if ((rc = ensure(dctl, 3 * kSrvGateOffset))) return rc;
+        if ((rc = ensure(dentries, kSrvRing * sizeof(SrvEntry)))) return rc;
+        if ((rc = ensure(tiles_done, S * sizeof(uint32_t)))) return rc;
+        if ((rc = ensure(hit_cursor, S * sizeof(unsigned long long)))) return rc;
+        if ((rc = ensure(partials, (size_t)S * cfg.max_tiles_per_request * sizeof(TilePartial))))
+            return rc;
+        fill_u32_kernel<<<8, 256, 0, kstream>>>((uint32_t*)dctl.p, 3 * kSrvGateOffset / 4, 0u);
+        fill_u32_kernel<<<64, 256, 0, kstream>>>((uint32_t*)tiles_done.p, S, 0u);
+        fill_u32_kernel<<<64, 256, 0, kstream>>>((uint32_t*)hit_cursor.p, 2ull * S, 0u);
+        PP_HIP(hipGetLastError());
+        PP_HIP(hipStreamSynchronize(kstream));
+        // the hit pool: device memory with result copies on an SDMA engine,
+        // else (or when asked for) pinned host memory
+        const uint64_t pool_bytes = std::max<uint64_t>(cfg.hit_pool_records, 1) * sizeof(pp_hit);
+        if (!(cfg.flags & PP_SERVER_HOST_POOL)) {
+            if ((rc = ensure(pool, pool_bytes))) return rc;
+            if (!setup_sdma()) {
+                (void)hipFree(pool.p);
+                pool.p = nullptr;
+                pool.cap = 0;
+            }
+        }
+        if (!sdma) PP_HIP(hipHostMalloc((void**)&host_pool, pool_bytes, hipHostMallocCoherent));
+        // the kernel's own stop after device_wait_us with nothing published
+        int rate_khz = 0;
+        PP_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device));
+        const uint64_t wait_us =
+            cfg.device_wait_us ? cfg.device_wait_us
+                               : std::max<uint64_t>(4ull * cfg.idle_exit_us, 1000000ull);
+        wait_ticks = std::max<uint64_t>(1, wait_us * (uint64_t)std::max(rate_khz, 1) / 1000);
+        SrvAddr A;
+        memset(&A, 0, sizeof A);
+        A.ctl = ctl;
+        A.entries = entries;
+        A.slots = slots;
+        A.done = done;
+        A.ticket = (unsigned long long*)dctl.p;
+        A.gate = (unsigned long long*)((char*)dctl.p + kSrvGateOffset);
+        A.dmeta = (unsigned long long*)((char*)dctl.p + 2 * kSrvGateOffset);
+        A.dentries = (SrvEntry*)dentries.p;
+        A.tiles_done = (uint32_t*)tiles_done.p;
+        A.hit_cursor = (unsigned long long*)hit_cursor.p;
+        A.partials = (TilePartial*)partials.p;
+        A.pool = host_pool ? host_pool : (pp_hit*)pool.p;
+        A.wait_ticks = wait_ticks;
+        A.max_tiles = cfg.max_tiles_per_request;
+        if ((rc = ensure(addr, sizeof A))) return rc;
+        PP_HIP(hipMemcpy(addr.p, &A, sizeof A, hipMemcpyHostToDevice));
+        // The grid leaves room beside it on every SIMD for the caller's other
+        // kernels: 32 VGPRs and one of the 8 wave slots.
+        int per_cu = 0;
+        PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, 0));
+        hipFuncAttributes fa;
+        PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
+        const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
+        per_cu = std::min({per_cu, (512 - kSrvVgprReserve) / vgpr, kSrvMaxWavesPerSimd});
+        grid_ = std::max(1, per_cu) * c->n_cu;
+        return PP_OK;
+    }
+};
 
 }  // namespace
+
+struct pp_server {
+    pps::ServerCore<HipBackend> core;
+    float n_min = 0.0f;  // smallest n of the server's table snapshot (Cherenkov check)
+    explicit pp_server(const pp_server_config& cfg) : core(cfg) {}
+};
 
 extern "C" {
 
@@ -2369,18 +2049,19 @@ int pp_server_create(pp_ctx* c, const pp_spectrum* spectrum, const pp_server_con
     if (cfg->max_tiles_per_request < 1 || cfg->max_tiles_per_request > (1u << 20))
         return fail(PP_ERR_INVALID_ARG, "max_tiles_per_request %u not in [1, 2^20]",
                     cfg->max_tiles_per_request);
-    pp_server* s = new pp_server();
-    s->ctx = c;
-    s->cfg = *cfg;
-    if (!s->cfg.idle_exit_us) s->cfg.idle_exit_us = 20000;
-    s->cfg.idle_exit_us = std::min<uint32_t>(s->cfg.idle_exit_us, 10000000u);
-    if ((rc = server_setup(s, spectrum))) {
+    if (cfg->flags & ~PP_SERVER_HOST_POOL)
+        return fail(PP_ERR_INVALID_ARG, "unknown server flags 0x%x", cfg->flags);
+    pp_server_config k = *cfg;
+    if (!k.idle_exit_us) k.idle_exit_us = 20000;
+    k.idle_exit_us = std::min<uint32_t>(k.idle_exit_us, 10000000u);
+    pp_server* s = new pp_server(k);
+    s->n_min = c->n_min;
+    if ((rc = s->core.be.setup(c, spectrum, k)) || (rc = s->core.start())) {
         std::string msg = last_error();
-        free_server(s);
+        delete s;
         last_error() = msg;
         return rc;
     }
-    s->worker = std::thread(worker_main, s);
     *out = s;
     return PP_OK;
 }
@@ -2392,129 +2073,34 @@ int pp_server_submit(pp_server* s, const pp_request* request, uint64_t hits_cap,
     int rc = make_devreq(request, 0, &dr);
     if (rc) return rc;
     // against the server's own table snapshot, not the context's current medium
-    if ((rc = check_cone(s->ctx, request, 1, s->n_min))) return rc;
-    const uint64_t n_tiles = (request->emitter.n_photons + kTilePhotons - 1) / kTilePhotons;
-    if (n_tiles > s->cfg.max_tiles_per_request)
-        return fail(PP_ERR_INVALID_ARG, "request of %llu photons exceeds %u tiles of %d",
-                    (unsigned long long)request->emitter.n_photons, s->cfg.max_tiles_per_request,
-                    PP_TILE_PHOTONS);
-    if (hits_cap > s->cfg.hit_pool_records)
-        return fail(PP_ERR_INVALID_ARG, "hits_cap %llu exceeds the hit pool (%llu records)",
-                    (unsigned long long)hits_cap, (unsigned long long)s->cfg.hit_pool_records);
-    std::lock_guard<std::mutex> g(s->mu);
-    if (!s->worker_error.empty()) return fail(PP_ERR_DEVICE, "%s", s->worker_error.c_str());
-    Job j;
-    j.ticket = s->next_ticket++;
-    j.req = *request;
-    j.hits_cap = n_tiles ? hits_cap : 0;  // a job without photons reserves no records
-    j.n_tiles = n_tiles;
-    j.t_submit = std::chrono::steady_clock::now();
-    s->jobs[j.ticket] = j;
-    s->pending.push_back(j.ticket);
-    ++s->submitted;
-    admit_locked(s);
-    s->cv.notify_all();
-    *ticket_out = j.ticket;
-    return PP_OK;
+    if ((rc = check_cone(request, 1, s->n_min))) return rc;
+    return s->core.submit(dr, hits_cap, ticket_out);
 }
 
 int pp_server_wait(pp_server* s, uint64_t ticket, uint32_t timeout_ms, int* done_out) {
     if (!s || !done_out) return fail(PP_ERR_INVALID_ARG, "NULL argument");
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(ticket);
-    if (it == s->jobs.end()) return fail(PP_ERR_NOT_FOUND, "unknown ticket %llu", (unsigned long long)ticket);
-    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
-    while (!it->second.done && s->worker_error.empty()) {
-        if (s->cv.wait_until(lk, deadline) == std::cv_status::timeout) break;
-        it = s->jobs.find(ticket);
-    }
-    if (!s->worker_error.empty()) return fail(PP_ERR_DEVICE, "%s", s->worker_error.c_str());
-    *done_out = it->second.done ? 1 : 0;
-    return PP_OK;
+    return s->core.wait(ticket, timeout_ms, done_out);
 }
 
 int pp_server_result(pp_server* s, uint64_t ticket, pp_summary* summary, pp_hit* hits_out,
                      uint64_t hits_cap, uint64_t* n_hits_out, double* latency_us) {
     if (!s) return fail(PP_ERR_INVALID_ARG, "server is NULL");
     if (hits_cap && !hits_out) return fail(PP_ERR_INVALID_ARG, "hits_out is NULL with hits_cap > 0");
-    Job j;
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        auto it = s->jobs.find(ticket);
-        if (it == s->jobs.end()) return fail(PP_ERR_NOT_FOUND, "unknown ticket %llu", (unsigned long long)ticket);
-        if (!it->second.done) return fail(PP_ERR_BUSY, "ticket %llu not finished", (unsigned long long)ticket);
-        j = it->second;
-    }
-    const uint64_t n = std::min<uint64_t>(j.summary.n_stored, hits_cap);
-    if (n && s->sdma) {
-        // pool -> pinned staging on the SDMA engine, staging -> caller on the
-        // host, the two halves alternating so that both overlap
-        std::lock_guard<std::mutex> cg(s->copy_mu);
-        const pp_hit* src = (const pp_hit*)s->pool.p + j.hit_base;
-        uint64_t off = 0;
-        int h = 0;
-        if (!sdma_issue(s, 0, src, std::min(n, kSrvStageRecords)))
-            return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed to start",
-                        (unsigned long long)ticket);
-        while (off < n) {
-            const uint64_t m = std::min(n - off, kSrvStageRecords);
-            if (!sdma_wait(s, h, 60000))
-                return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed",
-                            (unsigned long long)ticket);
-            const uint64_t next = off + m;
-            if (next < n && !sdma_issue(s, h ^ 1, src + next, std::min(n - next, kSrvStageRecords)))
-                return fail(PP_ERR_DEVICE, "SDMA copy of ticket %llu's records failed to start",
-                            (unsigned long long)ticket);
-            parallel_memcpy(reinterpret_cast<char*>(hits_out + off),
-                            reinterpret_cast<const char*>(s->stage + (uint64_t)h * kSrvStageRecords),
-                            m * sizeof(pp_hit));
-            off = next;
-            h ^= 1;
-        }
-    } else if (n) {
-        PP_HIP(hipSetDevice(s->ctx->device));
-        PP_HIP(hipMemcpyAsync(hits_out, (pp_hit*)s->pool.p + j.hit_base, n * sizeof(pp_hit),
-                              hipMemcpyDeviceToHost, s->cstream));
-        PP_HIP(hipStreamSynchronize(s->cstream));
-    }
-    if (summary) *summary = j.summary;
-    if (n_hits_out) *n_hits_out = n;
-    if (latency_us)
-        *latency_us = std::chrono::duration<double, std::micro>(j.t_done - j.t_submit).count();
-    std::lock_guard<std::mutex> g(s->mu);
-    pool_release(s, j.hit_base, j.hits_cap);
-    s->jobs.erase(ticket);
-    admit_locked(s);
-    s->cv.notify_all();
-    return PP_OK;
+    return s->core.result(ticket, summary, hits_out, hits_cap, n_hits_out, latency_us);
 }
 
 int pp_server_stats_get(pp_server* s, pp_server_stats* out) {
     if (!s || !out) return fail(PP_ERR_INVALID_ARG, "NULL argument");
-    std::lock_guard<std::mutex> g(s->mu);
-    out->submitted = s->submitted;
-    out->completed = s->completed;
-    out->pending = s->pending.size();
-    out->in_flight = s->submitted - s->completed - s->pending.size();
-    out->kernel_launches = s->launches;
-    out->pool_records_free = s->pool_free_total;
-    out->tiles_published = s->published;
-    out->grid_blocks = (uint64_t)s->grid;
+    s->core.stats(out);
+    out->host_pool = s->core.be.host_pool ? 1 : 0;
     return PP_OK;
 }
 
+// Drain: stop admitting, let the kernel finish in-flight work and exit.
 int pp_server_destroy(pp_server* s) {
     if (!s) return PP_OK;
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        s->shutdown = true;
-        s->cv.notify_all();
-    }
-    if (s->worker.joinable()) s->worker.join();
-    (void)hipSetDevice(s->ctx->device);
-    if (s->ctl) __atomic_store_n(&s->ctl->stop, 1ull, __ATOMIC_RELEASE);
-    if (s->kstream) (void)hipStreamSynchronize(s->kstream);
-    free_server(s);
+    s->core.shutdown();
+    delete s;
     return PP_OK;
 }
 

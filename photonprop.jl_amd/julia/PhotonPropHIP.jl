@@ -89,6 +89,8 @@ struct PPServerConfig
     max_tiles_per_request::UInt32
     hit_pool_records::UInt64
     idle_exit_us::UInt32
+    device_wait_us::UInt32
+    flags::UInt32
     reserved::UInt32
 end
 
@@ -112,7 +114,7 @@ const EMIT_CONE, EMIT_UNIFORM, EMIT_REF_ISOTROPIC = UInt32(0), UInt32(1), UInt32
 function __init__()
     @assert sizeof(PPHit) == 64 && sizeof(PPSummary) == 64 && sizeof(PPEmitter) == 48
     @assert sizeof(PPMediumTables) == 48 && sizeof(PPSpectrum) == 32
-    @assert sizeof(PPRequest) == 80 && sizeof(PPServerConfig) == 24
+    @assert sizeof(PPRequest) == 80 && sizeof(PPServerConfig) == 32
 end
 
 last_error() = unsafe_string(ccall((:pp_last_error, LIB), Cstring, ()))
@@ -240,7 +242,7 @@ end
 
 """
     HIPServer(ctx, spectrum; max_slots=256, max_tiles_per_request=16384,
-              hit_pool_records=2^24, idle_exit_us=20000)
+              hit_pool_records=2^24, idle_exit_us=20000, device_wait_us=0, host_pool=false)
 
 Continuous batching of propagation requests into a persistent kernel.  Queue
 with `submit!`, poll with `wait_done`, collect with `take_result!`.
@@ -251,9 +253,11 @@ mutable struct HIPServer
     keep::Any
     function HIPServer(ctx::HIPContext, spectrum::Spectrum; max_slots::Integer=256,
                        max_tiles_per_request::Integer=16384, hit_pool_records::Integer=1 << 24,
-                       idle_exit_us::Integer=20000)
+                       idle_exit_us::Integer=20000, device_wait_us::Integer=0,
+                       host_pool::Bool=false)
         cfg = PPServerConfig(UInt32(max_slots), UInt32(max_tiles_per_request),
-                             UInt64(hit_pool_records), UInt32(idle_exit_us), 0)
+                             UInt64(hit_pool_records), UInt32(idle_exit_us),
+                             UInt32(device_wait_us), host_pool ? UInt32(1) : UInt32(0), 0)
         keep, spec = spectrum_arg(spectrum)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         GC.@preserve keep check(ccall((:pp_server_create, LIB), Cint,

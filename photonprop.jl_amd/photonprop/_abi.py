@@ -101,12 +101,17 @@ class pp_request(ctypes.Structure):
     ]
 
 
+PP_SERVER_HOST_POOL = 1
+
+
 class pp_server_config(ctypes.Structure):
     _fields_ = [
         ("max_slots", ctypes.c_uint32),
         ("max_tiles_per_request", ctypes.c_uint32),
         ("hit_pool_records", ctypes.c_uint64),
         ("idle_exit_us", ctypes.c_uint32),
+        ("device_wait_us", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
     ]
 
@@ -114,7 +119,8 @@ class pp_server_config(ctypes.Structure):
 class pp_server_stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "completed", "in_flight", "pending", "kernel_launches",
-        "pool_records_free", "tiles_published", "grid_blocks")]
+        "pool_records_free", "tiles_published", "grid_blocks", "device_stops",
+        "kernel_running", "host_pool")]
 
 
 PP_TILE_PHOTONS = 8192

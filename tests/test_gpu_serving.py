@@ -338,3 +338,60 @@ def test_server_serving_workload_full_size(ctx, rate):
             say("request %d against the oracle:" % i, cmp)
             assert osum["n_hits"] > 1000
             assert_parity(cmp, osum["n_hits"])
+
+
+def test_server_kernel_stops_itself_after_device_wait(ctx):
+    """The kernel's own stop: with the host's idle stop 10 s away, the kernel
+    stops by itself once it has waited device_wait_us = 3 ms with nothing
+    published.  It is then no longer running, the stop is counted, and a
+    request submitted next runs on a relaunched kernel (resumed at the tile
+    the kernel reported) with the same result as alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(2, seed=88)
+    reqs["n_photons"] = [250_000, 180_000]
+    with pp.Server(ctx, spec, max_slots=4, max_tiles_per_request=64, hit_pool_records=1 << 20,
+                   idle_exit_us=10_000_000, device_wait_us=3000) as srv:
+        ta = srv.submit(reqs[0], 250_000)
+        assert srv.wait(ta, 60000)
+        sa, ha, _ = srv.result(ta, 250_000)
+        time.sleep(0.1)  # far beyond 3 ms, far below the host's 10 s
+        st = srv.stats()
+        say("after the device wait:", st)
+        assert st["device_stops"] >= 1 and st["kernel_running"] == 0
+        tb = srv.submit(reqs[1], 180_000)
+        assert srv.wait(tb, 60000)
+        sb, hb, _ = srv.result(tb, 180_000)
+        st = srv.stats()
+    assert st["kernel_launches"] >= 2 and st["completed"] == 2
+    for r, s, h in ((reqs[0], sa, ha), (reqs[1], sb, hb)):
+        ref = alone(ctx, r)
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (k, s[k], ref.summary[k])
+        np.testing.assert_array_equal(key(h), key(ref.hits))
+
+
+def test_server_host_pool_results_equal_alone(ctx):
+    """The hit pool in pinned host memory (PP_SERVER_HOST_POOL; the mode used
+    without an SDMA engine): the kernel writes the records straight to the
+    host, no device-to-host copy is involved, and every result equals the same
+    request alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(12, seed=91)
+    caps = [int(n) for n in reqs["n_photons"]]
+    with pp.Server(ctx, spec, max_slots=4, max_tiles_per_request=512, hit_pool_records=1 << 22,
+                   host_pool=True) as srv:
+        assert srv.stats()["host_pool"] == 1
+        tickets = [srv.submit(r, caps[i]) for i, r in enumerate(reqs)]
+        got = []
+        for i, t in enumerate(tickets):
+            assert srv.wait(t, 60000), i
+            got.append(srv.result(t, caps[i]))
+        st = srv.stats()
+    say("host-pool stats", st)
+    assert st["completed"] == len(reqs)
+    for i, r in enumerate(reqs):
+        s, h, _ = got[i]
+        ref = alone(ctx, r)
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (i, k, s[k], ref.summary[k])
+        np.testing.assert_array_equal(key(h), key(ref.hits))
