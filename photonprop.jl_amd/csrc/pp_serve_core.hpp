@@ -301,10 +301,10 @@ private:
     void publish_locked() {
         if (stopping_) return;  // tickets restart at resume_tile_ on the next launch
         // Claim reports are plain stores from many CUs and may land out of
-        // order; the highest seen is kept.  Should the estimate stay full with
-        // no new report for 1 ms (an older report landing last), everything
-        // published is taken as claimed: that can only publish early, never
-        // stall the kernel.
+        // order; the highest seen is kept.  Should the estimate stay full for
+        // 1 ms with neither a new report nor a new chunk (an older report
+        // landing last), everything published is taken as claimed: that can
+        // only publish early, never stall the kernel.
         const auto now = std::chrono::steady_clock::now();
         const uint64_t target =
             std::max<uint64_t>(kSrvBacklogGrids * be.grid(), 4 * kSrvChunkTiles);
@@ -334,6 +334,7 @@ private:
                 std::max(kSrvChunkTiles, (j.n_tiles + kSrvMaxChunks - 1) / kSrvMaxChunks);
             publish_entry(j, std::min(chunk, j.n_tiles - j.next_local));
             if (j.next_local == j.n_tiles) large_q_.pop_front();
+            claim_change_ = now;  // the 1 ms without reports counts from the last chunk
         }
         if (n_entries_ != n_before) {
             std::atomic_thread_fence(std::memory_order_seq_cst);
