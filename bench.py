@@ -237,13 +237,18 @@ def bench_serving(pp, ctx, spectrum, n_req):
     }
 
 
-def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, n_eval=512, eval_photons=20_000_000):
-    """Train the BF16-MFMA surrogate on MC rows from the batched kernel, then evaluate
-    it against high-statistics MC on held-out pairs: median relative yield error
-    (pairs with >= 400 weighted hits, i.e. <= 5% MC noise) and the median
-    Kolmogorov distance between the predicted gamma-mixture CDF and the MC
-    residual-time sample.  Speed: surrogate queries/s (one forward of 1e6 rows,
-    HIP events) against MC queries/s at the 4096-pair batch's mean budget."""
+def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log_std=0.05,
+                    n_eval=1024, eval_photons=100_000_000):
+    """Train the BF16-MFMA surrogate (an ensemble of `members` networks on bootstrap
+    resamples of the MC rows from the batched kernel), then evaluate it on
+    held-out pairs against high-statistics MC: yield errors over every pair with
+    a truth hit, next to the truth's own error bar, and over the pairs resolved
+    to <= 5 %; the variance gate's fallback fraction, the error of the pairs it
+    keeps and the share of the worst-error decile it sends to MC; the hybrid's
+    answers; the median Kolmogorov distance between the predicted arrival-time
+    mixture and the MC residual times.  Speed: ensemble queries/s (forward of
+    1e6 rows per member, HIP events) against MC queries/s at the 4096-pair
+    batch's mean budget."""
     import torch
     from scipy.special import gammainc
 
@@ -258,68 +263,57 @@ def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, n_eval=512, eval_p
     out["datagen_s"] = time.perf_counter() - t0
     print("surrogate: %d rows generated in %.1f s" % (n_rows, out["datagen_s"]), file=sys.stderr,
           flush=True)
-    mlp = sg.MLP(ctx, seed=0)
     t0 = time.perf_counter()
-    hist = sg.train(mlp, ts, epochs=epochs, batch=4096, lr=3e-3, time_weight=0.5,
-                    log=lambda m: print("surrogate " + m, file=sys.stderr, flush=True))
+    ens, hists = sg.train_ensemble(ctx, ts, k=members, epochs=epochs, batch=4096, lr=3e-3,
+                                   time_weight=0.5, seed=0)
     out["train_s"] = time.perf_counter() - t0
+    out["members"] = members
     out["epochs"] = epochs
-    out["final_loss"] = hist[-1]
+    out["final_loss"] = [h[-1] for h in hists]
+    print("surrogate: %d networks trained in %.1f s" % (members, out["train_s"]), file=sys.stderr,
+          flush=True)
     del ts
-    # held-out evaluation in 4 media
-    rng = np.random.default_rng(1234)
-    yp, ym, ks_d, fallback = [], [], [], 0
-    per = n_eval // 4
-    for g in range(4):
-        ms = np.array([rng.uniform(*sg.DOMAIN[k]) for k in ("abs_450", "sca_450", "g", "n_offset")])
-        medium = sg.medium_for(*ms)
-        raw, reqs = sg.random_rows(rng, per, ms)
-        ctx.set_medium(medium)
-        reqs["n_photons"] = eval_photons
-        reqs["seed"] = 777 + g
-        reqs["stream"] = np.arange(per, dtype=np.uint32)
-        hits, off, sums = ctx.propagate_batch(reqs, pp.InverseSquareSpectrum(300.0, 600.0),
-                                              per_request_cap=2000, hits_cap=1 << 26,
-                                              allow_overflow=True)
-        y = mlp.forward(sg.scale_features(raw))
-        mc = sums["sum_w"] / eval_photons
-        good = sums["sum_w"] >= 400.0
-        yp.append(np.exp(y[good, 0].astype(np.float64)))
-        ym.append(mc[good])
-        w, a, b = sg.gamma_mixture(y)
-        tg = sg.t_geo(np.linalg.norm(reqs["center"], axis=1), reqs["radius"], medium)
-        for r in np.nonzero(good)[0]:
-            t = np.sort(hits["time"][int(off[r]):int(off[r + 1])].astype(np.float64) - tg[r])
+    pairs = sg.held_out_pairs(ctx, n_eval, eval_photons, seed=1234, per_request_cap=2000)
+    hy = sg.HybridYield(ens, ctx, max_log_std=max_log_std, mc_photons=10_000_000)
+    _, rep = sg.evaluate_hybrid(hy, pairs, eval_photons)
+    out["eval_photons_per_pair"] = eval_photons
+    out["eval"] = rep
+    ks_d = []
+    for p in pairs:
+        _, _, ys = ens.predict(sg.scale_features(p["raw"]))
+        w, a, b = ens.time_mixture(ys)
+        tg = sg.t_geo(np.linalg.norm(p["reqs"]["center"], axis=1), p["reqs"]["radius"], p["medium"])
+        off = p["offsets"]
+        for r in np.nonzero(p["sums"]["sum_w"] >= 400.0)[0]:
+            t = np.sort(p["hits"]["time"][int(off[r]):int(off[r + 1])].astype(np.float64) - tg[r])
             t = np.maximum(t, 0.0) + 1e-2
             if t.size < 50:
                 continue
             cdf = (w[r][None, :] * gammainc(a[r][None, :], b[r][None, :] * t[:, None])).sum(1)
             emp = np.arange(1, t.size + 1) / t.size
             ks_d.append(float(np.max(np.abs(cdf - emp))))
-        fallback += int((np.exp(y[:, 0]) < 1e-7).sum() + (~sg.in_domain(raw)).sum())
-    yp, ym = np.concatenate(yp), np.concatenate(ym)
-    rel = np.abs(yp - ym) / ym
-    out["eval_pairs_resolved"] = int(rel.size)
-    out["median_yield_rel_error"] = float(np.median(rel))
-    out["p90_yield_rel_error"] = float(np.percentile(rel, 90))
     out["median_time_ks_distance"] = float(np.median(ks_d)) if ks_d else None
-    out["eval_fallback_fraction"] = fallback / float(4 * per)
+    del pairs
     ctx.set_medium(pp.SeawaterConfigMedium())
-    # speed: surrogate forward on 1e6 queries vs MC per query
+    # speed: ensemble forward on 1e6 queries vs MC per query
     n_q = 1_000_000
     xq = torch.rand((n_q, 8), device="cuda:0") * 2 - 1
     yq = torch.empty((n_q, 10), device="cuda:0")
-    mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n_q)
-    ms = min(mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n_q) for _ in range(3))
+    torch.cuda.synchronize()  # the library's stream does not wait for torch's
+    ms = 0.0
+    for m in ens.members:
+        m.forward_device(xq.data_ptr(), yq.data_ptr(), n_q)
+        ms += min(m.forward_device(xq.data_ptr(), yq.data_ptr(), n_q) for _ in range(3))
     out["surrogate_queries_per_s"] = n_q / (1e-3 * ms)
-    out["surrogate_tflops"] = n_q * 2.0 * (8 * 256 + 3 * 256 * 256 + 256 * 10) / (1e-3 * ms) / 1e12
+    out["surrogate_tflops"] = (members * n_q * 2.0 * (8 * 256 + 3 * 256 * 256 + 256 * 10)
+                               / (1e-3 * ms) / 1e12)
     reqs = workloads.batched_pairs(4096, seed=0)
     ctx.propagate_batch(reqs, pp.InverseSquareSpectrum(300.0, 600.0), hits_cap=0, allow_overflow=True)
     kms = ctx.last_kernel_ms()
     out["mc_queries_per_s"] = 4096 / (1e-3 * kms)
     out["mc_mean_photons_per_query"] = float(reqs["n_photons"].mean())
     out["speedup_vs_mc"] = out["surrogate_queries_per_s"] / out["mc_queries_per_s"]
-    mlp.close()
+    ens.close()
     return out
 
 
@@ -340,6 +334,8 @@ def main():
                     help="MC training rows for the surrogate measurement (0 skips it)")
     ap.add_argument("--surrogate-photons", type=float, default=1e5, help="photons per training row")
     ap.add_argument("--surrogate-epochs", type=int, default=40)
+    ap.add_argument("--surrogate-members", type=int, default=4,
+                    help="networks in the ensemble whose spread is the predictive uncertainty")
     ap.add_argument("--mlp-probe", type=float, default=0,
                     help="only time the surrogate's fused forward on this many random rows "
                          "(random weights; for profiling) and exit")
@@ -369,6 +365,7 @@ def main():
         mlp = sg.MLP(ctx, seed=0)
         xq = torch.rand((n, 8), device="cuda:0") * 2 - 1
         yq = torch.empty((n, 10), device="cuda:0")
+        torch.cuda.synchronize()  # the library's stream does not wait for torch's
         times = [mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n) for _ in range(5)]
         fl = n * 2.0 * (8 * 256 + 3 * 256 * 256 + 256 * 10)
         print(json.dumps({"mlp_probe_rows": n, "forward_ms": times,
@@ -481,7 +478,8 @@ def main():
         print("serving done", file=sys.stderr, flush=True)
         if args.surrogate_rows > 0:
             out["surrogate"] = bench_surrogate(pp, ctx, int(args.surrogate_rows),
-                                               int(args.surrogate_photons), args.surrogate_epochs)
+                                               int(args.surrogate_photons), args.surrogate_epochs,
+                                               members=args.surrogate_members)
             print("surrogate done", file=sys.stderr, flush=True)
     if args.cpu_sample > 0:
         cb = cpu_baseline(pp, int(args.cpu_sample))

@@ -19,8 +19,10 @@ directly on MC rows with the likelihoods themselves:
 
 Training, inference and the loss run in libphotonprop.so (pp_mlp_*:
 hand-written BF16 MFMA kernels, FP32 accumulation).  Queries outside the
-training domain, or where the predicted yield is below the floor the training
-MC could resolve, fall back to the plain MC kernel.
+training domain, where the predicted yield is below the floor the training MC
+could resolve, or where an ensemble of networks trained on bootstrap
+resamples disagrees by more than a set spread of the log yield (the
+predictive uncertainty), fall back to the plain MC kernel.
 """
 import ctypes
 import math
@@ -226,45 +228,135 @@ class MLP:
         return loss
 
 
-def train(mlp, ts, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0, log=None):
+def device_data(ts, dev="cuda:0"):
+    """The training set's arrays as device tensors (torch as plumbing)."""
+    import torch
+
+    return tuple(torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                 for a in (ts.x, ts.log_n, ts.k, ts.times))
+
+
+def train(mlp, ts, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0, log=None, rows=None,
+          data=None):
     """Adam over shuffled minibatches held in device memory (torch tensors as
-    plumbing); cosine learning-rate decay.  Returns per-epoch mean losses."""
+    plumbing); cosine learning-rate decay.  `rows` (optional) is the multiset of
+    training-row indices to draw from (a bootstrap resample); `data` the
+    tensors of device_data(ts), shared between networks.  Returns per-epoch
+    mean losses."""
     import torch
 
     dev = "cuda:%d" % 0
+    X, LN, K, T = data if data is not None else device_data(ts, dev)
+    base = (torch.arange(X.shape[0]) if rows is None
+            else torch.from_numpy(np.asarray(rows, dtype=np.int64)))
+    n = int(base.shape[0])
     # start the log-yield output at the data's mean yield (the output layer is
     # initialised small, so y0 would otherwise start at log(1) and take
     # thousands of Adam steps to reach yields of 1e-4..1e-8)
+    sel = base.numpy()
     p = mlp.get_params()
-    p[mlp.n_params - mlp.n_out] = math.log(max(float(ts.k.sum()), 1e-30) /
-                                           float(np.exp(ts.log_n.astype(np.float64)).sum()))
+    p[mlp.n_params - mlp.n_out] = math.log(
+        max(float(ts.k[sel].astype(np.float64).sum()), 1e-30) /
+        float(np.exp(ts.log_n[sel].astype(np.float64)).sum()))
     mlp.set_params(p)
-    X = torch.from_numpy(ts.x).to(dev)
-    LN = torch.from_numpy(ts.log_n).to(dev)
-    K = torch.from_numpy(ts.k).to(dev)
-    T = torch.from_numpy(ts.times).to(dev)
-    n = X.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed)
     steps_total = epochs * (n // batch)
     step = 0
     hist = []
+    row_bytes = [t.stride(0) * t.element_size() for t in (X, LN, K, T)]
     for ep in range(epochs):
-        perm = torch.randperm(n, generator=g).to(dev)
+        perm = base[torch.randperm(n, generator=g)].to(dev)
+        # one gather per epoch; minibatches are contiguous row blocks of it
+        Xp, LNp, Kp, Tp = X[perm], LN[perm], K[perm], T[perm]
+        ptrs = [t.data_ptr() for t in (Xp, LNp, Kp, Tp)]
+        # the library's stream does not wait for torch's: finish the gather first
+        torch.cuda.current_stream().synchronize()
         acc = np.zeros(2)
         nb = 0
         for b in range(n // batch):
-            idx = perm[b * batch:(b + 1) * batch]
-            xb, lb, kb, tb = (X[idx].contiguous(), LN[idx].contiguous(), K[idx].contiguous(),
-                              T[idx].contiguous())
+            o = b * batch
             cur = lr * 0.5 * (1 + math.cos(math.pi * step / max(1, steps_total)))
-            acc += mlp.train_step(xb.data_ptr(), lb.data_ptr(), kb.data_ptr(), tb.data_ptr(),
+            acc += mlp.train_step(*(q + o * rb for q, rb in zip(ptrs, row_bytes)),
                                   T.shape[1], batch, cur, time_weight)
             nb += 1
             step += 1
+        del Xp, LNp, Kp, Tp
         hist.append((acc / max(nb, 1)).tolist())
         if log:
             log("epoch %d: poisson %.5f time %.5f" % (ep, hist[-1][0], hist[-1][1]))
     return hist
+
+
+class Ensemble:
+    """K surrogate networks trained on bootstrap resamples of one MC training
+    set.  The prediction is the members' mean log yield; their spread (sample
+    standard deviation of the log yield) is the predictive uncertainty that
+    HybridYield gates on."""
+
+    def __init__(self, members):
+        self.members = list(members)
+
+    @property
+    def k(self):
+        return len(self.members)
+
+    def forward_all(self, x):
+        """(K, n, 10) outputs of every member."""
+        return np.stack([m.forward(x) for m in self.members])
+
+    def predict(self, x):
+        """(mean log yield, its spread across members, all outputs)."""
+        ys = self.forward_all(x)
+        ly = ys[:, :, 0].astype(np.float64)
+        std = ly.std(axis=0, ddof=1) if self.k > 1 else np.zeros(ly.shape[1])
+        return ly.mean(axis=0), std, ys
+
+    def time_mixture(self, ys):
+        """The members' arrival-time mixtures as one mixture of 3K gamma
+        components with weights summing to 1: (weights, shapes, rates per ns)."""
+        parts = [gamma_mixture(y) for y in ys]
+        w = np.concatenate([p[0] for p in parts], axis=1) / len(parts)
+        return w, np.concatenate([p[1] for p in parts], axis=1), np.concatenate([p[2] for p in parts], axis=1)
+
+    def close(self):
+        for m in self.members:
+            m.close()
+
+
+def train_ensemble(ctx, ts, k=4, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0,
+                   bootstrap=True, log=None):
+    """Train k networks (different initialisation and order; each on its own
+    bootstrap resample of the rows when `bootstrap`) concurrently, one host
+    thread and one stream per network.  Returns (Ensemble, per-member loss
+    histories)."""
+    import threading
+
+    data = device_data(ts)
+    n = ts.x.shape[0]
+    rng = np.random.default_rng(seed + 17)
+    members = [MLP(ctx, seed=seed * 7919 + 101 * i) for i in range(k)]
+    rows = [rng.integers(0, n, n) if bootstrap else None for _ in range(k)]
+    hists, errors = [None] * k, []
+
+    def run(i):
+        try:
+            lg = (lambda m: log("member %d %s" % (i, m))) if log else None
+            hists[i] = train(members[i], ts, epochs=epochs, batch=batch, lr=lr,
+                             time_weight=time_weight, seed=seed * 31 + i, log=lg, rows=rows[i],
+                             data=data)
+        except Exception as e:  # re-raised on the calling thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(k)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        for m in members:
+            m.close()
+        raise errors[0]
+    return Ensemble(members), hists
 
 
 def gamma_mixture(y):
@@ -280,26 +372,126 @@ def gamma_mixture(y):
 
 class HybridYield:
     """Yield per emitted photon from the surrogate, with the plain MC as the
-    fallback for rows out of the training domain or below the resolvable
-    yield floor."""
+    fallback for rows out of the training domain, below the resolvable yield
+    floor, or (for an Ensemble) whose predicted spread of the log yield exceeds
+    max_log_std."""
 
-    def __init__(self, mlp, ctx, yield_floor=1e-7, mc_photons=10_000_000, spectrum=None):
-        self.mlp = mlp
+    def __init__(self, model, ctx, yield_floor=1e-7, max_log_std=0.05, mc_photons=10_000_000,
+                 spectrum=None):
+        self.model = model
         self.ctx = ctx
         self.floor = yield_floor
+        self.max_log_std = max_log_std
         self.mc_photons = mc_photons
         self.spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
 
-    def query(self, raw, reqs, medium):
+    def predict(self, raw):
+        """Surrogate only: (yield, spread of the log yield, MC-fallback mask)."""
+        x = scale_features(raw)
+        if isinstance(self.model, Ensemble):
+            ly, std, _ = self.model.predict(x)
+        else:
+            ly = self.model.forward(x)[:, 0].astype(np.float64)
+            std = np.zeros_like(ly)
+        yld = np.exp(ly)
+        use_mc = ~in_domain(raw) | (yld < self.floor) | (std > self.max_log_std)
+        return yld, std, use_mc
+
+    def query(self, raw, reqs, medium, seed=None):
         """raw (n, 8) physical features and the matching requests (same medium).
-        Returns (yield, used_mc mask)."""
-        y = self.mlp.forward(scale_features(raw))
-        yld = np.exp(y[:, 0].astype(np.float64))
-        use_mc = ~in_domain(raw) | (yld < self.floor)
+        Returns (yield, used_mc mask, spread of the surrogate's log yield)."""
+        yld, std, use_mc = self.predict(raw)
         if use_mc.any():
             self.ctx.set_medium(medium)
             r = reqs[use_mc].copy()
             r["n_photons"] = self.mc_photons
+            if seed is not None:
+                r["seed"] = seed
             _, _, sums = self.ctx.propagate_batch(r, self.spectrum, hits_cap=0, allow_overflow=True)
             yld[use_mc] = sums["sum_w"] / self.mc_photons
-        return yld, use_mc
+        return yld, use_mc, std
+
+
+def held_out_pairs(ctx, n_pairs, photons, seed=1234, n_media=4, per_request_cap=0, spectrum=None):
+    """High-statistics MC on held-out pairs: n_pairs random geometries spread
+    over n_media randomly drawn media, photons each.  Returns a list per medium
+    of dicts {medium, raw, reqs, hits, offsets, sums}; hits are kept up to
+    per_request_cap per pair."""
+    spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
+    rng = np.random.default_rng(seed)
+    out = []
+    per = n_pairs // n_media
+    for g in range(n_media):
+        ms = np.array([rng.uniform(*DOMAIN[k]) for k in ("abs_450", "sca_450", "g", "n_offset")])
+        medium = medium_for(*ms)
+        raw, reqs = random_rows(rng, per, ms)
+        ctx.set_medium(medium)
+        reqs["n_photons"] = photons
+        reqs["seed"] = seed * 1009 + g
+        reqs["stream"] = np.arange(per, dtype=np.uint32)
+        hits, off, sums = ctx.propagate_batch(
+            reqs, spectrum, per_request_cap=per_request_cap,
+            hits_cap=max(1, per_request_cap * per), allow_overflow=True)
+        out.append(dict(medium=medium, raw=raw, reqs=reqs, hits=hits, offsets=off, sums=sums))
+    return out
+
+
+def evaluate_hybrid(hy, pairs, truth_photons, mc_seed=4242):
+    """Surrogate and hybrid yields of held-out pairs against their MC truth.
+
+    Per pair: truth yield Σw/N, its relative error bar 1/sqrt(Σw) (an upper
+    bound: every weight is <= 1, so Var Σw <= Σw), the surrogate yield and
+    spread, the gate's decision and the hybrid's answer (the MC fallback runs
+    with its own seed).  Returns (arrays, summary)."""
+    cols = {k: [] for k in ("truth", "truth_err", "sumw", "surrogate", "spread", "gated",
+                            "hybrid", "in_domain", "below_floor")}
+    for p in pairs:
+        sur, std, gate = hy.predict(p["raw"])
+        hyb, used, _ = hy.query(p["raw"], p["reqs"], p["medium"], seed=mc_seed)
+        sw = p["sums"]["sum_w"].astype(np.float64)
+        cols["truth"].append(sw / truth_photons)
+        cols["truth_err"].append(np.where(sw > 0, 1.0 / np.sqrt(np.maximum(sw, 1e-300)), np.inf))
+        cols["sumw"].append(sw)
+        cols["surrogate"].append(sur)
+        cols["spread"].append(std)
+        cols["gated"].append(used)
+        cols["hybrid"].append(hyb)
+        cols["in_domain"].append(in_domain(p["raw"]))
+        cols["below_floor"].append(sur < hy.floor)
+    a = {k: np.concatenate(v) for k, v in cols.items()}
+    ok = a["sumw"] > 0
+    res = a["sumw"] >= 400.0  # truth known to <= 5 %
+    rel_s = np.full(ok.shape, np.inf)
+    rel_h = np.full(ok.shape, np.inf)
+    rel_s[ok] = np.abs(a["surrogate"][ok] - a["truth"][ok]) / a["truth"][ok]
+    rel_h[ok] = np.abs(a["hybrid"][ok] - a["truth"][ok]) / a["truth"][ok]
+    a["rel_surrogate"], a["rel_hybrid"] = rel_s, rel_h
+    worst = res & (rel_s >= np.percentile(rel_s[res], 90)) if res.any() else res
+    keep = res & ~a["gated"]
+    s = {
+        "pairs": int(ok.size),
+        "pairs_without_truth_hits": int((~ok).sum()),
+        "pairs_resolved": int(res.sum()),
+        # surrogate alone, every pair with a truth hit, with the truth's own error bar
+        "surrogate_median_rel_error_all": float(np.median(rel_s[ok])),
+        "surrogate_p90_rel_error_all": float(np.percentile(rel_s[ok], 90)),
+        "truth_median_rel_error_bar_all": float(np.median(a["truth_err"][ok])),
+        "surrogate_within_2sigma_plus_5pct": float(np.mean(
+            rel_s[ok] <= 2.0 * a["truth_err"][ok] + 0.05)),
+        "surrogate_median_rel_error_resolved": float(np.median(rel_s[res])),
+        "surrogate_p90_rel_error_resolved": float(np.percentile(rel_s[res], 90)),
+        # the gate
+        "max_log_std": hy.max_log_std,
+        "fallback_fraction": float(a["gated"].mean()),
+        "fallback_out_of_domain": float((~a["in_domain"]).mean()),
+        "fallback_below_floor": float((a["in_domain"] & a["below_floor"]).mean()),
+        "fallback_spread": float((a["in_domain"] & ~a["below_floor"] & a["gated"]).mean()),
+        "kept_resolved_pairs": int(keep.sum()),
+        "kept_median_rel_error": float(np.median(rel_s[keep])) if keep.any() else None,
+        "kept_p90_rel_error": float(np.percentile(rel_s[keep], 90)) if keep.any() else None,
+        "worst_decile_sent_to_mc": float(a["gated"][worst].mean()) if worst.any() else None,
+        # the hybrid's answers
+        "hybrid_median_rel_error_resolved": float(np.median(rel_h[res])),
+        "hybrid_median_rel_error_all": float(np.median(rel_h[ok])),
+    }
+    return a, s

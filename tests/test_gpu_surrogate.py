@@ -172,6 +172,35 @@ def test_mc_rows_train_and_hybrid_fallback():
         raw, reqs = sg.random_rows(rng, 8, ms)
         raw[0, 0] = 3.0  # 1 km: outside the training domain
         reqs["seed"] = 1
-        y, used = hy.query(raw, reqs, sg.medium_for(*ms))
+        y, used, _ = hy.query(raw, reqs, sg.medium_for(*ms))
         assert used[0] and np.all(np.isfinite(y)) and np.all(y >= 0)
         mlp.close()
+
+
+def test_ensemble_gate_against_held_out_mc():
+    """The surrogate's acceptance on held-out pairs, at a reduced training size:
+    four networks trained on bootstrap resamples of 2e5 MC rows, 1024 held-out
+    pairs in 4 media against 1e8-photon MC.  The gate (out of domain, below the
+    yield floor, or spread of the members' log yields above 0.05) must keep the
+    surrogate's own answers at a median yield error <= 5 %, send at least 90 %
+    of the worst-error decile to MC, and leave the hybrid's answers at a median
+    error <= 5 %."""
+    with pp.Context(pp.SeawaterConfigMedium()) as c:
+        ts = sg.generate_training_set(c, 200_000, photons_per_row=100_000, n_times=16, seed=3)
+        ens, hists = sg.train_ensemble(c, ts, k=4, epochs=40, seed=0)
+        del ts
+        assert all(h[-1][0] < h[0][0] for h in hists)
+        pairs = sg.held_out_pairs(c, 1024, 100_000_000, seed=1234)
+        hy = sg.HybridYield(ens, c, max_log_std=0.05, mc_photons=100_000_000)
+        a, s = sg.evaluate_hybrid(hy, pairs, 100_000_000)
+        say(s)
+        assert s["pairs_resolved"] >= 300
+        assert s["kept_resolved_pairs"] >= 10
+        assert s["kept_median_rel_error"] <= 0.05
+        assert s["worst_decile_sent_to_mc"] >= 0.9
+        assert s["hybrid_median_rel_error_resolved"] <= 0.05
+        # a wider gate keeps a superset of the pairs
+        hy.max_log_std = 0.2
+        _, _, wide = hy.predict(np.concatenate([p["raw"] for p in pairs]))
+        assert np.all(a["gated"] >= wide)
+        ens.close()
