@@ -221,11 +221,17 @@ static void FN(or_photon_)(const pp_medium_tables* med, const pp_spectrum* sp, c
      *   block 0:      λ <- u0, emission θ <- u1, emission φ <- p0;
      *                 step 0 <- {u2, u3, p1};
      *   block b >= 1: step 2b-1 <- {u0, u1, p0}, step 2b <- {u2, u3, p1};
-     * {step length, scattering angle, azimuth} per step. */
+     * {step length, scattering angle, azimuth} per step.  The final step
+     * (max_steps - 1) uses only its length; when it is the first step of its
+     * block and the emission is the Cherenkov cone (which does not use u1 of
+     * block 0), its length is u1 of block 0 (the device then never computes
+     * that block). */
     uint32_t w0[4];
     float ub[4], pb[2];
     or_philox4x32_10(ctr, key, w0);
     or_block_draws(w0, ub, pb);
+    const float u_lone = ub[1];
+    const int lone_from_block0 = em->profile == PP_EMIT_CHERENKOV_CONE && max_steps % 2u == 0u;
 
     REAL u_lambda = (REAL)ub[0];
     REAL lambda = FN(or_wavelength_)(sp, knots, u_lambda);
@@ -256,13 +262,23 @@ static void FN(or_photon_)(const pp_medium_tables* med, const pp_spectrum* sp, c
     for (uint32_t s = 0; s < max_steps; ++s) {
         const uint32_t blk = (s + 1u) / 2u;
         const uint32_t slot = (s % 2u == 0u) ? 1u : 0u; /* even steps: {u2, u3, p1} */
-        if (blk != have_block) {
-            ctr[0] = blk;
-            or_philox4x32_10(ctr, key, words);
-            or_block_draws(words, ub, pb);
-            have_block = blk;
+        REAL u[3];
+        if (lone_from_block0 && s + 1u == max_steps) {
+            /* the final step, first of its block: only its length is drawn */
+            u[0] = (REAL)u_lone;
+            u[1] = (REAL)0.5;
+            u[2] = (REAL)0;
+        } else {
+            if (blk != have_block) {
+                ctr[0] = blk;
+                or_philox4x32_10(ctr, key, words);
+                or_block_draws(words, ub, pb);
+                have_block = blk;
+            }
+            u[0] = (REAL)ub[2 * slot];
+            u[1] = (REAL)ub[2 * slot + 1];
+            u[2] = (REAL)pb[slot];
         }
-        REAL u[3] = {(REAL)ub[2 * slot], (REAL)ub[2 * slot + 1], (REAL)pb[slot]};
         ++steps;
         REAL step = -R_LOG(u[0]) * sca; /* src/photon_prop_cuda.jl:322-323 */
         REAL d;

@@ -197,6 +197,16 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, flo
     oz = rz * inv;
 }
 
+// Renormalise a direction of squared length 1 + e, |e| of a few 1e-6: one
+// Newton step of 1/sqrt from 1 (error 3e²/8, far below an FP32 ulp).
+__device__ __forceinline__ void renormalize(float& x, float& y, float& z) {
+    const float n2 = __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
+    const float inv = __builtin_fmaf(-0.5f, n2, 1.5f);
+    x *= inv;
+    y *= inv;
+    z *= inv;
+}
+
 // Linear interpolation in a uniform-grid table, clamped (same operation order
 // as the oracle's or_table_).  The medium table interleaves {n, λ_sca, λ_abs, 0}.
 __device__ __forceinline__ float4 table_lerp(const float4* t, uint32_t n, float xs) {

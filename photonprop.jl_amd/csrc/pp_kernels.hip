@@ -238,8 +238,13 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         //                 step 0 <- {a(z), a(w), p(z, w)}
         //   block b >= 1: step 2b-1 <- {a(x), a(y), p(x, y)},
         //                 step 2b   <- {a(z), a(w), p(z, w)}
-        // with {step length, scattering angle, azimuth} per step.
+        // with {step length, scattering angle, azimuth} per step.  The final
+        // step (max_steps - 1) needs only its length: when it is the first
+        // step of its block and the emission is the Cherenkov cone (which
+        // leaves a(y) of block 0 unused), its length is a(y) of block 0 and
+        // its block is never computed.
         const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+        const bool cone = R.profile == PP_EMIT_CHERENKOV_CONE;
         const float u_l = 2.0f - ppd::draw_a(w0.x);
         float lambda;
         if (T.spec.kind == PP_SPEC_MONO) {
@@ -326,33 +331,47 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             const float st = ppd::sin_from_cos(ct);                                     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
-            ppd::rotate_to_axis<RN, true, true>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz); \
+            ppd::rotate_to_axis<false, true, true>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz); \
+            if (RN) ppd::renormalize(dx, dy, dz);                                       \
         }                                                                               \
     }
 
         // No per-step ballot: a wave whose lanes have all hit skips the
         // (divergent) step bodies and Philox blocks on an empty exec mask.
-        // The direction is renormalised after step 0 and after steps 4, 8, ...
-        // (at most four rotations apart; the emission direction is left to
-        // step 0).  Two Philox blocks per iteration, one per pair of steps.
-        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), true,
+        // The direction is renormalised after every eighth rotation, counting
+        // the emission as the first (after steps 6, 14, 22, ...): a unit
+        // vector's length moves by ~2e-7 per rotation, so it stays within a
+        // few 1e-6 of 1, where one Newton step renormalises exactly.  Two
+        // Philox blocks per iteration, one per pair of steps.
+        const uint32_t w_lone = w0.y;  // the lone final step's length word (cone)
+        PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), false,
                 max_steps == 1u)
         for (uint32_t s0 = 1; s0 < max_steps; s0 += 4) {
             ppd::U4 Q;
-            if (alive)
-                Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            if (alive) {
+                if (cone && s0 + 1 >= max_steps)
+                    Q = ppd::U4{w_lone, w_lone, w_lone, w_lone};
+                else
+                    Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo,
+                                      R.seed_hi);
+            }
             PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false,
                     s0 + 1 >= max_steps)
             if (s0 + 1 >= max_steps) break;
-            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), false,
-                    s0 + 2 >= max_steps)
+            PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w),
+                    (s0 & 7u) == 5u, s0 + 2 >= max_steps)
             if (s0 + 2 >= max_steps) break;
-            if (alive)
-                Q = ppd::philox10((s0 + 3u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            if (alive) {
+                if (cone && s0 + 3 >= max_steps)
+                    Q = ppd::U4{w_lone, w_lone, w_lone, w_lone};
+                else
+                    Q = ppd::philox10((s0 + 3u) >> 1, id_lo, id_hi, R.stream, R.seed_lo,
+                                      R.seed_hi);
+            }
             PP_STEP(s0 + 2u, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false,
                     s0 + 3 >= max_steps)
             if (s0 + 3 >= max_steps) break;
-            PP_STEP(s0 + 3u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), true,
+            PP_STEP(s0 + 3u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), false,
                     s0 + 4 >= max_steps)
         }
 #undef PP_STEP
