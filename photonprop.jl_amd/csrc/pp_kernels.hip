@@ -201,7 +201,8 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                                                 uint32_t count, pp_hit* hits, uint64_t hits_cap,
                                                 unsigned long long* hit_cursor,
                                                 double (*s_red)[3], uint64_t (*s_redi)[2],
-                                                uint4 (*s_hitbuf)[kHitBuf * 4]) {
+                                                uint4 (*s_hitbuf)[kHitBuf * 4],
+                                                float (*s_idir)[kBlock]) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
@@ -288,7 +289,11 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         }
         // (dy carries a bias of -2^-60, which the rotations need at the exact
         // south pole; it leaves every |y| above 2^-36 unchanged)
-        const float idx = dx, idy = dy, idz = dz;
+        // the emission direction waits in LDS for the hit record (registers
+        // stay free for the step loop)
+        s_idir[0][tid] = dx;
+        s_idir[1][tid] = dy;
+        s_idir[2][tid] = dz;
         // position relative to the target centre: the sphere test needs no
         // per-step subtraction; the absolute position is rebuilt for a hit
         float qx = R.pos[0] - cx, qy = R.pos[1] - cy, qz = R.pos[2] - cz;
@@ -397,8 +402,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
                                             __float_as_uint(pz), __float_as_uint(t));
                 const uint4 q1 = make_uint4(__float_as_uint(dx), __float_as_uint(dy),
                                             __float_as_uint(dz), __float_as_uint(lambda));
-                const uint4 q2 = make_uint4(__float_as_uint(idx), __float_as_uint(idy),
-                                            __float_as_uint(idz), __float_as_uint(wgt));
+                const uint4 q2 = make_uint4(__float_as_uint(s_idir[0][tid]),
+                                            __float_as_uint(s_idir[1][tid]),
+                                            __float_as_uint(s_idir[2][tid]), __float_as_uint(wgt));
                 const uint4 q3 = make_uint4(__float_as_uint(dist), req_index, id_lo, id_hi);
                 uint4* o = hbuf + (fill + rank) * 4;
                 o[0] = q0;
@@ -442,12 +448,13 @@ __device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevR
                                                     uint64_t hits_cap,
                                                     unsigned long long* hit_cursor,
                                                     double (*s_red)[3], uint64_t (*s_redi)[2],
-                                                    uint4 (*s_hitbuf)[kHitBuf * 4]) {
+                                                    uint4 (*s_hitbuf)[kHitBuf * 4],
+                                                    float (*s_idir)[kBlock]) {
     if (T.hg.iso)
         return run_tile<kSysStores, true>(T, R, req_index, first, count, hits, hits_cap,
-                                          hit_cursor, s_red, s_redi, s_hitbuf);
+                                          hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
     return run_tile<kSysStores, false>(T, R, req_index, first, count, hits, hits_cap,
-                                       hit_cursor, s_red, s_redi, s_hitbuf);
+                                       hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
 }
 
 // The per-launch constants of run_tile.
@@ -470,6 +477,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ double s_red[kWaves][3];
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
+    __shared__ float s_idir[3][kBlock];
     const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     for (;;) {
         __syncthreads();  // previous tile's s_tile and s_red consumed
@@ -492,7 +500,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
         const TilePartial tp = run_tile_any<false>(T, R, r, first, count, P.hits, P.hits_cap,
-                                                   P.hit_cursor, s_red, s_redi, s_hitbuf);
+                                                   P.hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
         if (threadIdx.x == 0) P.partials[tile] = tp;
     }
 }
@@ -1590,6 +1598,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ double s_red[kWaves][3];
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
+    __shared__ float s_idir[3][kBlock];
     const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_desc_slot = ~0u;
@@ -1724,7 +1733,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         TilePartial* parts = SA(partials) + (uint64_t)slot * SA(max_tiles);
         const TilePartial tp = run_tile_any<true>(T, R, slot, first, count, SA(pool) + hit_base,
                                                   hit_cap, &SA(hit_cursor)[slot], s_red, s_redi,
-                                                  s_hitbuf);
+                                                  s_hitbuf, s_idir);
         // every wave waits for its own (system-scope) hit stores to complete
         // before the barrier; a workgroup-scope fence would not wait for them
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
