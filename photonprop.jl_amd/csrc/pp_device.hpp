@@ -207,6 +207,22 @@ __device__ __forceinline__ void renormalize(float& x, float& y, float& z) {
     z *= inv;
 }
 
+// Linear interpolation in the medium table, clamped.  Point i holds two
+// float4: {n, λ_sca·ln2, λ_abs, 0} at t[2i] and the differences to point i+1
+// (formed in FP32 on the host, as hi - lo would be here; zero at the last
+// point) at t[2i+1], so a lookup is one clamp (v_med3), one address and three
+// fmas, lo + f·(hi - lo) in the oracle's operation order (or_table_), with no
+// clamp of the index: xs = n-1 gives i = n-1 and f = 0.
+__device__ __forceinline__ float3 table_lerp_slopes(const float4* t, float xmax, float xs) {
+    xs = __builtin_amdgcn_fmed3f(xs, 0.0f, xmax);
+    const uint32_t i = (uint32_t)xs;
+    const float f = xs - (float)i;
+    const float4 v = t[2u * i];
+    const float4 d = t[2u * i + 1u];
+    return make_float3(__builtin_fmaf(f, d.x, v.x), __builtin_fmaf(f, d.y, v.y),
+                       __builtin_fmaf(f, d.z, v.z));
+}
+
 // Linear interpolation in a uniform-grid table, clamped (same operation order
 // as the oracle's or_table_).  The medium table interleaves {n, λ_sca, λ_abs, 0}.
 __device__ __forceinline__ float4 table_lerp(const float4* t, uint32_t n, float xs) {

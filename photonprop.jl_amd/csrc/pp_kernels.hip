@@ -68,19 +68,25 @@ struct TilePartial {
     double sum_w, sum_wt, sum_wt2;
 };
 
+// The per-launch constants of run_tile (kernel-argument memory: run_tile
+// reads them with scalar loads where it uses them).
+struct TileCtx {
+    const float4* table;  // medium table, two float4 per point (device memory; read once
+                          // per photon, cached)
+    const float* knots;   // spectrum knots (device memory)
+    float xmax;           // n_points - 1
+    float lmin, inv_dl;
+    ppd::HG hg;
+    SpecParams spec;
+};
+
 struct KParams {
     const DevReq* reqs;
     const uint64_t* tile_start;  // [n_req+1]
     uint32_t n_req;
-    uint32_t n_points;
     uint64_t n_tiles;
     uint32_t* tile_counter;
-    const float4* table;
-    const float* knots;
-    float lmin;
-    float inv_dl;
-    ppd::HG hg;
-    SpecParams spec;
+    TileCtx tile;
     pp_hit* hits;
     uint64_t hits_cap;
     unsigned long long* hit_cursor;
@@ -102,6 +108,25 @@ __device__ __forceinline__ T load_constant(const T* p, uint64_t i) {
 #else
     return p[i];
 #endif
+}
+
+// The same for a pointer the compiler may hold in vector registers (the value
+// is wave-uniform; readfirstlane moves it to scalar registers first).
+template <typename T>
+__device__ __forceinline__ const T* opaque_uniform_ptr(const T* p) {
+    const uint64_t v = (uint64_t)p;
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    __asm__ volatile("" : "+s"(lo), "+s"(hi));
+    return (const T*)(((uint64_t)hi << 32) | lo);
+}
+
+// Pointer p, opaque to the compiler at this point: a load through it is not
+// hoisted or merged with another, so the loaded value lives only from here.
+template <typename T>
+__device__ __forceinline__ const T* opaque_ptr(const T* p) {
+    __asm__ volatile("" : "+s"(p));
+    return p;
 }
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
@@ -179,14 +204,6 @@ __device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit*
     }
 }
 
-struct TileCtx {
-    const float4* table;  // medium table (device memory; read once per photon, cached)
-    const float* knots;   // spectrum knots (device memory)
-    uint32_t n_points;
-    float lmin, inv_dl;
-    ppd::HG hg;
-    SpecParams spec;
-};
 
 // Photons [first, first + count) of request R on one 256-lane block: emit,
 // step, append hits (record field `request` = req_index), and reduce the
@@ -196,7 +213,7 @@ struct TileCtx {
 // (= T.hg.iso, uniform per launch) takes the isotropic-scattering test out of
 // the step loop; run_tile_any picks the instance.
 template <bool kSysStores, bool kIso>
-__device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& R,
+__device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq& R,
                                                 uint32_t req_index, uint64_t first,
                                                 uint32_t count, pp_hit* hits, uint64_t hits_cap,
                                                 unsigned long long* hit_cursor,
@@ -221,6 +238,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
     const uint32_t max_steps = R.max_steps;
     const float r2 = R.r2;
     const float cx = R.center[0], cy = R.center[1], cz = R.center[2];
+    const ppd::HG hg = load_constant(&Tc->hg, 0);  // the step loop's constants
 
     // steps of this lane's photons: at most kRoundsPerTile * max_steps <= 2^29
     // (max_steps <= 2^24); the wave's sum is taken in 64 bits
@@ -248,6 +266,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
         const bool cone = R.profile == PP_EMIT_CHERENKOV_CONE;
         const float u_l = 2.0f - ppd::draw_a(w0.x);
         float lambda;
+        // the launch constants of the emission, loaded for this round only (no
+        // registers held for them through the step loop)
+        const TileCtx T = load_constant(opaque_uniform_ptr(Tc), 0);
         if (T.spec.kind == PP_SPEC_MONO) {
             lambda = T.spec.mono;
         } else if (T.spec.kind == PP_SPEC_KNOTS) {
@@ -257,9 +278,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             // division sequence, here and for the cone angle below
             lambda = __builtin_amdgcn_rcpf(__builtin_fmaf(-u_l, T.spec.d, T.spec.a));
         }
-        const float4 med = ppd::table_lerp(T.table, T.n_points, (lambda - T.lmin) * T.inv_dl);
+        const float3 med = ppd::table_lerp_slopes(T.table, T.xmax, (lambda - T.lmin) * T.inv_dl);
         const float n_phase = med.x;
-        const float sca_ln2 = med.y * kLn2;
+        const float sca_ln2 = med.y;  // the table holds λ_sca·ln2
         const float abs_len = med.z;
 
         float dx, dy, dz;
@@ -332,7 +353,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
             hit = true;                                                                 \
             alive = false;                                                              \
         } else if (!(LAST)) {                                                           \
-            const float ct = ppd::hg_costheta_a_t<kIso>(T.hg, (AC));                    \
+            const float ct = ppd::hg_costheta_a_t<kIso>(hg, (AC));                      \
             const float st = ppd::sin_from_cos(ct);                                     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
@@ -442,7 +463,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx& T, const DevReq& 
 }
 
 template <bool kSysStores>
-__device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevReq& R,
+__device__ __forceinline__ TilePartial run_tile_any(const TileCtx* Tc, const DevReq& R,
                                                     uint32_t req_index, uint64_t first,
                                                     uint32_t count, pp_hit* hits,
                                                     uint64_t hits_cap,
@@ -450,26 +471,23 @@ __device__ __forceinline__ TilePartial run_tile_any(const TileCtx& T, const DevR
                                                     double (*s_red)[3], uint64_t (*s_redi)[2],
                                                     uint4 (*s_hitbuf)[kHitBuf * 4],
                                                     float (*s_idir)[kBlock]) {
-    if (T.hg.iso)
-        return run_tile<kSysStores, true>(T, R, req_index, first, count, hits, hits_cap,
+    if (load_constant(&Tc->hg, 0).iso)
+        return run_tile<kSysStores, true>(Tc, R, req_index, first, count, hits, hits_cap,
                                           hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
-    return run_tile<kSysStores, false>(T, R, req_index, first, count, hits, hits_cap,
+    return run_tile<kSysStores, false>(Tc, R, req_index, first, count, hits, hits_cap,
                                        hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
 }
 
-// The per-launch constants of run_tile.
-__device__ __forceinline__ TileCtx make_tile_ctx(const float4* table, uint32_t n_points,
-                                                 const float* knots, const SpecParams& spec,
-                                                 float lmin, float inv_dl, const ppd::HG& hg) {
-    TileCtx T;
-    T.table = table;
-    T.knots = knots;
-    T.n_points = n_points;
-    T.lmin = lmin;
-    T.inv_dl = inv_dl;
-    T.hg = hg;
-    T.spec = spec;
-    return T;
+
+// P.tile where the kernel arguments lie (P is the kernel's only argument, at
+// offset 0): taking &P instead would make the compiler copy P to private memory.
+__device__ __forceinline__ const TileCtx* kernarg_tile() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (const TileCtx*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() +
+                            offsetof(KParams, tile));
+#else
+    return nullptr;  // host pass: device code is only parsed
+#endif
 }
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) propagate_kernel(KParams P) {
@@ -478,7 +496,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     __shared__ float s_idir[3][kBlock];
-    const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     for (;;) {
         __syncthreads();  // previous tile's s_tile and s_red consumed
         if (threadIdx.x == 0) s_tile = atomicAdd(P.tile_counter, 1u);
@@ -499,7 +516,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t first = ((uint64_t)tile - load_constant(P.tile_start, r)) * kTilePhotons;
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
-        const TilePartial tp = run_tile_any<false>(T, R, r, first, count, P.hits, P.hits_cap,
+        const TilePartial tp = run_tile_any<false>(kernarg_tile(), R, r, first, count, P.hits, P.hits_cap,
                                                    P.hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
         if (threadIdx.x == 0) P.partials[tile] = tp;
     }
@@ -760,12 +777,35 @@ __global__ void __launch_bounds__(256) selftest_hg_moments_kernel(ppd::HG hg, ui
 namespace {
 
 
+// The per-launch constants of run_tile, as the kernels' arguments carry them.
+TileCtx make_tile_ctx(const float4* table, uint32_t n_points, const float* knots,
+                      const SpecParams& spec, float lmin, float inv_dl, const ppd::HG& hg) {
+    TileCtx T;
+    T.table = table;
+    T.knots = knots;
+    T.xmax = (float)(n_points - 1u);
+    T.lmin = lmin;
+    T.inv_dl = inv_dl;
+    T.hg = hg;
+    T.spec = spec;
+    return T;
+}
+
 int upload_medium(pp_ctx* c, const pp_medium_tables* m) {
     int rc = check_medium(m);
     if (rc) return rc;
-    std::vector<float4> t(m->n_points);
-    for (uint32_t i = 0; i < m->n_points; ++i)
-        t[i] = make_float4(m->n_phase[i], m->sca_len[i], m->abs_len[i], 0.0f);
+    // point i: {n, λ_sca·ln2, λ_abs, 0}, then the FP32 differences to point
+    // i+1 (zero at the last point), as the kernel's table_lerp_slopes reads it
+    const uint32_t n = m->n_points;
+    std::vector<float> sca(n);
+    for (uint32_t i = 0; i < n; ++i) sca[i] = m->sca_len[i] * kLn2;
+    std::vector<float4> t(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t j = i + 1 < n ? i + 1 : i;
+        t[2 * i] = make_float4(m->n_phase[i], sca[i], m->abs_len[i], 0.0f);
+        t[2 * i + 1] = make_float4(m->n_phase[j] - m->n_phase[i], sca[j] - sca[i],
+                                   m->abs_len[j] - m->abs_len[i], 0.0f);
+    }
     if ((rc = ensure(c->table, t.size() * sizeof(float4)))) return rc;
     PP_HIP(hipMemcpy(c->table.p, t.data(), t.size() * sizeof(float4), hipMemcpyHostToDevice));
     c->n_points = m->n_points;
@@ -928,15 +968,10 @@ int enqueue_propagation(pp_ctx* c, uint32_t n_req, uint64_t n_tiles, const SpecP
     P.reqs = W.reqs;
     P.tile_start = W.tile_start;
     P.n_req = n_req;
-    P.n_points = c->n_points;
     P.n_tiles = n_tiles;
     P.tile_counter = W.counters;
-    P.table = (const float4*)c->table.p;
-    P.knots = W.knots;
-    P.lmin = c->lmin;
-    P.inv_dl = c->inv_dl;
-    P.hg = make_hg(c->g);
-    P.spec = sp;
+    P.tile = make_tile_ctx((const float4*)c->table.p, c->n_points, W.knots, sp, c->lmin,
+                           c->inv_dl, make_hg(c->g));
     P.hits = W.staging;
     P.hits_cap = hits_cap;
     P.hit_cursor = (unsigned long long*)(W.counters + 2);
@@ -1490,25 +1525,13 @@ struct SrvAddr {
     pp_hit* pool;                     // device memory, or pinned host memory
     uint64_t wait_ticks;              // s_memrealtime ticks the head waits before stopping
     uint64_t max_tiles;
+    TileCtx tile;                     // the server's medium, spectrum and scattering
 };
 
 struct SrvParams {
     const SrvAddr* addr;
-    uint32_t n_points;
-    const float4* table;
-    const float* knots;
-    float lmin, inv_dl;
-    ppd::HG hg;
-    SpecParams spec;
 };
 
-// Pointer p, opaque to the compiler at this point: a load through it is not
-// hoisted or merged with another, so the loaded value lives only from here.
-template <typename T>
-__device__ __forceinline__ const T* opaque_ptr(const T* p) {
-    __asm__ volatile("" : "+s"(p));
-    return p;
-}
 
 // A field of the serving addresses, read here with a scalar load.
 #define SA(field) load_constant(&opaque_ptr(P.addr)->field, 0)
@@ -1599,7 +1622,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     __shared__ float s_idir[3][kBlock];
-    const TileCtx T = make_tile_ctx(P.table, P.n_points, P.knots, P.spec, P.lmin, P.inv_dl, P.hg);
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_desc_slot = ~0u;
 
@@ -1731,7 +1753,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t hit_base = uniform_u64(sd->hit_base);
         const uint64_t seq = uniform_u64(sd->seq);
         TilePartial* parts = SA(partials) + (uint64_t)slot * SA(max_tiles);
-        const TilePartial tp = run_tile_any<true>(T, R, slot, first, count, SA(pool) + hit_base,
+        const TilePartial tp = run_tile_any<true>(&opaque_ptr(P.addr)->tile, R, slot, first, count, SA(pool) + hit_base,
                                                   hit_cap, &SA(hit_cursor)[slot], s_red, s_redi,
                                                   s_hitbuf, s_idir);
         // every wave waits for its own (system-scope) hit stores to complete
@@ -1868,13 +1890,6 @@ struct HipBackend {
         SrvParams P;
         memset(&P, 0, sizeof P);
         P.addr = (const SrvAddr*)addr.p;
-        P.n_points = n_points;
-        P.table = (const float4*)table.p;
-        P.knots = (const float*)knots.p;
-        P.lmin = lmin;
-        P.inv_dl = inv_dl;
-        P.hg = make_hg(g);
-        P.spec = spec;
         // every tile below resume_tile is done and none above it was started
         serve_reset_kernel<<<1, 64, 0, kstream>>>(
             (unsigned long long*)dctl.p, (unsigned long long*)((char*)dctl.p + kSrvGateOffset),
@@ -1977,8 +1992,8 @@ struct HipBackend {
         if ((rc = upload_knots(spectrum, (float*)knots.p, c->stream))) return rc;
         // the server keeps its own copy of the medium table (pp_set_medium on
         // the context does not change requests already being served)
-        if ((rc = ensure(table, (size_t)c->n_points * sizeof(float4)))) return rc;
-        PP_HIP(hipMemcpyAsync(table.p, c->table.p, (size_t)c->n_points * sizeof(float4),
+        if ((rc = ensure(table, 2 * (size_t)c->n_points * sizeof(float4)))) return rc;
+        PP_HIP(hipMemcpyAsync(table.p, c->table.p, 2 * (size_t)c->n_points * sizeof(float4),
                               hipMemcpyDeviceToDevice, c->stream));
         n_points = c->n_points;
         lmin = c->lmin;
@@ -2040,6 +2055,8 @@ struct HipBackend {
         A.pool = host_pool ? host_pool : (pp_hit*)pool.p;
         A.wait_ticks = wait_ticks;
         A.max_tiles = cfg.max_tiles_per_request;
+        A.tile = make_tile_ctx((const float4*)table.p, n_points, (const float*)knots.p, spec,
+                               lmin, inv_dl, make_hg(g));
         if ((rc = ensure(addr, sizeof A))) return rc;
         PP_HIP(hipMemcpy(addr.p, &A, sizeof A, hipMemcpyHostToDevice));
         // The grid leaves room beside it on every SIMD for the caller's other
