@@ -267,6 +267,10 @@ int pp_last_kernel_ms(pp_ctx* ctx, float* ms);
                                    straight to the host and results are copied out by the
                                    host alone.  Used by itself when the device has no SDMA
                                    engine to copy results from device memory. */
+#define PP_SERVER_LEAVE_ROOM 2u /* the persistent grid takes 7 of the 8 wave slots and leaves
+                                   32 VGPRs on every SIMD, so the caller's other kernels run
+                                   beside a busy server (about 3 % lower serving throughput);
+                                   by default it takes the whole device */
 
 typedef struct pp_server_config {
     uint32_t max_slots;             /* requests on the device at once, 1..4096 */

@@ -1497,14 +1497,14 @@ using pps::SrvSlot;
 constexpr uint64_t kSrvExit = ~0ull;
 constexpr unsigned long long kGateStop = 1ull << 63;  // gate word: published | stop
 constexpr size_t kSrvGateOffset = 4096;  // ticket, gate and mirror counts on separate pages
-// Room left on every SIMD for other kernels: VGPRs per lane and wave slots (of
-// 8) the persistent grid does not take.  Result copies never need compute
-// units (an SDMA engine, or a hit pool in host memory), so this is for the
-// caller's other work on the device.  64 photons x 1e8 through the server took
-// 111.0, 101.4 and 99.2 ms at 5, 7 and 8 waves per SIMD (the batch kernel
-// 99.2 ms); 7 keeps one wave slot free on every SIMD.
+// Room left on every SIMD for other kernels under PP_SERVER_LEAVE_ROOM: VGPRs
+// per lane and wave slots (of 8) the persistent grid does not take.  Result
+// copies never need compute units (an SDMA engine, or a hit pool in host
+// memory), so this is only for the caller's other work on the device.  64
+// requests of 1e8 photons through the server took 95.2 ms at 8 waves per SIMD
+// and 97.9 ms at 7 (the batch kernel 94.1 ms).
 constexpr int kSrvVgprReserve = 32;
-constexpr int kSrvMaxWavesPerSimd = 7;
+constexpr int kSrvRoomWavesPerSimd = 7;
 constexpr uint64_t kSrvStageRecords = 1u << 16;  // 4 MiB per half of the result staging
 
 // The serving kernel's addresses and limits, in device memory (written once
@@ -2059,14 +2059,15 @@ struct HipBackend {
                                lmin, inv_dl, make_hg(g));
         if ((rc = ensure(addr, sizeof A))) return rc;
         PP_HIP(hipMemcpy(addr.p, &A, sizeof A, hipMemcpyHostToDevice));
-        // The grid leaves room beside it on every SIMD for the caller's other
-        // kernels: 32 VGPRs and one of the 8 wave slots.
+        // The grid fills the device, or leaves room beside it on every SIMD for
+        // the caller's other kernels (32 VGPRs and one of the 8 wave slots).
         int per_cu = 0;
         PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBlock, 0));
         hipFuncAttributes fa;
         PP_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(serve_kernel)));
         const int vgpr = std::max(8, (fa.numRegs + 7) & ~7);  // allocation granule of 8
-        per_cu = std::min({per_cu, (512 - kSrvVgprReserve) / vgpr, kSrvMaxWavesPerSimd});
+        if (cfg.flags & PP_SERVER_LEAVE_ROOM)
+            per_cu = std::min({per_cu, (512 - kSrvVgprReserve) / vgpr, kSrvRoomWavesPerSimd});
         grid_ = std::max(1, per_cu) * c->n_cu;
         return PP_OK;
     }
@@ -2094,7 +2095,7 @@ int pp_server_create(pp_ctx* c, const pp_spectrum* spectrum, const pp_server_con
     if (cfg->max_tiles_per_request < 1 || cfg->max_tiles_per_request > (1u << 20))
         return fail(PP_ERR_INVALID_ARG, "max_tiles_per_request %u not in [1, 2^20]",
                     cfg->max_tiles_per_request);
-    if (cfg->flags & ~PP_SERVER_HOST_POOL)
+    if (cfg->flags & ~(PP_SERVER_HOST_POOL | PP_SERVER_LEAVE_ROOM))
         return fail(PP_ERR_INVALID_ARG, "unknown server flags 0x%x", cfg->flags);
     pp_server_config k = *cfg;
     if (!k.idle_exit_us) k.idle_exit_us = 20000;

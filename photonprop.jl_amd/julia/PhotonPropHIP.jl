@@ -242,7 +242,8 @@ end
 
 """
     HIPServer(ctx, spectrum; max_slots=256, max_tiles_per_request=16384,
-              hit_pool_records=2^24, idle_exit_us=20000, device_wait_us=0, host_pool=false)
+              hit_pool_records=2^24, idle_exit_us=20000, device_wait_us=0, host_pool=false,
+              leave_room=false)
 
 Continuous batching of propagation requests into a persistent kernel.  Queue
 with `submit!`, poll with `wait_done`, collect with `take_result!`.
@@ -254,10 +255,11 @@ mutable struct HIPServer
     function HIPServer(ctx::HIPContext, spectrum::Spectrum; max_slots::Integer=256,
                        max_tiles_per_request::Integer=16384, hit_pool_records::Integer=1 << 24,
                        idle_exit_us::Integer=20000, device_wait_us::Integer=0,
-                       host_pool::Bool=false)
+                       host_pool::Bool=false, leave_room::Bool=false)
         cfg = PPServerConfig(UInt32(max_slots), UInt32(max_tiles_per_request),
                              UInt64(hit_pool_records), UInt32(idle_exit_us),
-                             UInt32(device_wait_us), host_pool ? UInt32(1) : UInt32(0), 0)
+                             UInt32(device_wait_us),
+                             (host_pool ? UInt32(1) : UInt32(0)) | (leave_room ? UInt32(2) : UInt32(0)), 0)
         keep, spec = spectrum_arg(spectrum)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         GC.@preserve keep check(ccall((:pp_server_create, LIB), Cint,

@@ -102,6 +102,7 @@ class pp_request(ctypes.Structure):
 
 
 PP_SERVER_HOST_POOL = 1
+PP_SERVER_LEAVE_ROOM = 2
 
 
 class pp_server_config(ctypes.Structure):

@@ -19,15 +19,17 @@ from .propagation import REQUEST_DTYPE, spectrum_struct, summary_dict
 class Server:
     def __init__(self, ctx, spectrum, max_slots=256, max_tiles_per_request=16384,
                  hit_pool_records=1 << 24, idle_exit_us=20000, device_wait_us=0,
-                 host_pool=False):
+                 host_pool=False, leave_room=False):
         """device_wait_us: the kernel stops by itself after waiting this long with
         nothing published (0: the library's default); host_pool: keep the hit
-        pool in pinned host memory (PP_SERVER_HOST_POOL)."""
+        pool in pinned host memory (PP_SERVER_HOST_POOL); leave_room: leave a
+        wave slot and 32 VGPRs per SIMD for other kernels (PP_SERVER_LEAVE_ROOM)."""
         self.ctx = ctx
         self.lib = ctx.lib
         cfg = _abi.pp_server_config(max_slots, max_tiles_per_request, int(hit_pool_records),
                                     int(idle_exit_us), int(device_wait_us),
-                                    _abi.PP_SERVER_HOST_POOL if host_pool else 0, 0)
+                                    (_abi.PP_SERVER_HOST_POOL if host_pool else 0) |
+                                    (_abi.PP_SERVER_LEAVE_ROOM if leave_room else 0), 0)
         spec, keep = spectrum_struct(spectrum)
         self._h = ctypes.c_void_p()
         _abi.check(self.lib.pp_server_create(ctx.handle, ctypes.byref(spec), ctypes.byref(cfg),

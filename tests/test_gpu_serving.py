@@ -395,3 +395,32 @@ def test_server_host_pool_results_equal_alone(ctx):
         for k in SUM_KEYS:
             assert s[k] == ref.summary[k], (i, k, s[k], ref.summary[k])
         np.testing.assert_array_equal(key(h), key(ref.hits))
+
+
+def test_server_leave_room_grid_and_results(ctx):
+    """PP_SERVER_LEAVE_ROOM: the persistent grid takes at most 7 of 8 wave
+    slots per SIMD (a smaller grid than the default, which fills the device),
+    and requests still equal the same request alone."""
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    reqs = requests(6, seed=93)
+    caps = [int(n) for n in reqs["n_photons"]]
+    grids = {}
+    for leave in (False, True):
+        with pp.Server(ctx, spec, max_slots=4, max_tiles_per_request=512,
+                       hit_pool_records=1 << 22, leave_room=leave) as srv:
+            tickets = [srv.submit(r, caps[i]) for i, r in enumerate(reqs)]
+            got = []
+            for i, t in enumerate(tickets):
+                assert srv.wait(t, 60000), i
+                got.append(srv.result(t, caps[i]))
+            grids[leave] = srv.stats()["grid_blocks"]
+        for i, r in enumerate(reqs):
+            s, h, _ = got[i]
+            ref = alone(ctx, r)
+            for k in SUM_KEYS:
+                assert s[k] == ref.summary[k], (leave, i, k)
+            np.testing.assert_array_equal(key(h), key(ref.hits))
+    _, n_cu = ctx.device_info()
+    say("grid blocks: full %d, leave_room %d (%d CUs)" % (grids[False], grids[True], n_cu))
+    assert grids[True] < grids[False]
+    assert grids[True] <= 7 * n_cu
