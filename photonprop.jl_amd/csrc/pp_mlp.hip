@@ -34,13 +34,15 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMaxLayers = 16;
+constexpr uint32_t kMaxSplits = 64;  // split-K partials of a weight gradient at most
+constexpr float kLogMuCap = 30.0f;   // Poisson log mean beyond which the loss grows quadratically
 constexpr uint32_t kTimeScale = 10;  // residual times enter the mixture in units of 10 ns
 
 inline uint32_t pad16(uint32_t x) { return (x + 15u) & ~15u; }
 
 enum Epilogue : int {
     EPI_STORE = 0,      // C = acc (fp32)
-    EPI_ATOMIC = 1,     // C += acc (fp32 atomics; split-K)
+    EPI_PARTIAL = 1,    // C[split] = acc (fp32; split-K partials, summed in order afterwards)
     EPI_BIAS_SILU = 2,  // C = z = acc + bias (fp32), Y = silu(z) (bf16)
     EPI_SILU_ONLY = 3,  // Y = silu(acc + bias) (bf16), C untouched
     EPI_BIAS = 4,       // C = acc + bias (fp32)
@@ -54,6 +56,7 @@ struct GemmArgs {
     bf16* Y;
     uint32_t lda, ldb, ldc, ldy;
     uint32_t M, N, K, k_split;
+    uint64_t split_stride;  // EPI_PARTIAL: floats between the partials of consecutive splits
     int epi;
 };
 
@@ -112,7 +115,9 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
                 const float v = acc[a][b][q];
                 switch (g.epi) {
                 case EPI_STORE: g.C[(size_t)i * g.ldc + j] = v; break;
-                case EPI_ATOMIC: atomicAdd(&g.C[(size_t)i * g.ldc + j], v); break;
+                case EPI_PARTIAL:
+                    g.C[(size_t)blockIdx.z * g.split_stride + (size_t)i * g.ldc + j] = v;
+                    break;
                 case EPI_BIAS_SILU: {
                     const float z = v + bj;
                     g.C[(size_t)i * g.ldc + j] = z;
@@ -378,10 +383,16 @@ __global__ void loss_kernel(const float* Y, uint32_t ldy, const float* logn, con
     if (i < M) {
         const float* y = Y + (size_t)i * ldy;
         const float inv_m = 1.0f / (float)M;
-        // Poisson
-        const float lmu = fminf(y[0] + logn[i], 60.0f);
-        const float mu = __expf(lmu);
-        lp = mu - kobs[i] * lmu;
+        // Poisson NLL mu - k·l of the log mean l = y0 + log N.  Above
+        // l = kLogMuCap (mu = 1.1e13, far above any row's count) mu's growth
+        // continues linearly, e^c·(1 + (l - c)), so a step that overshoots
+        // keeps a bounded gradient (e^l would overflow Adam's second moment and
+        // freeze the network there) and the loss stays convex.
+        const float l = y[0] + logn[i];
+        const float ex = fmaxf(l - kLogMuCap, 0.0f);
+        const float ec = __expf(fminf(l, kLogMuCap));
+        const float mu = ec * (1.0f + ex);  // the derivative of the mean term
+        lp = ec * (1.0f + ex + 0.5f * ex * ex) - kobs[i] * l;
         g[0] = (mu - kobs[i]) * inv_m;
         // gamma mixture
         float logit[3], al[3], be[3], lal[3], lbe[3];
@@ -470,13 +481,18 @@ __global__ void sync_weights_kernel(const float* wt, uint32_t n_out, uint32_t n_
 }
 
 // Gradient of layer weights in [n_out][ldk] padded form -> packed [n_out][n_in].
-__global__ void unpack_grad_kernel(const float* gp, uint32_t n_out, uint32_t n_in, uint32_t ldk,
-                                   float* g) {
+// The weight gradient from its split-K partials, added in split order (the
+// same sum on every run, unlike atomics), unpadded into the packed vector.
+__global__ void unpack_grad_splits_kernel(const float* gp, uint32_t splits, uint64_t stride,
+                                          uint32_t n_out, uint32_t n_in, uint32_t ldk, float* g) {
     const uint64_t n = (uint64_t)n_out * n_in;
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t o = (uint32_t)(t / n_in), k = (uint32_t)(t % n_in);
-        g[t] = gp[(size_t)o * ldk + k];
+        const float* p = gp + (size_t)o * ldk + k;
+        float acc = p[0];
+        for (uint32_t sp = 1; sp < splits; ++sp) acc += p[sp * stride];
+        g[t] = acc;
     }
 }
 
@@ -543,7 +559,8 @@ int mlp_reserve(pp_mlp* m, uint32_t rows) {
 
 int gemm(pp_mlp* m, const bf16* A, uint32_t lda, const bf16* B, uint32_t ldb, uint32_t M,
          uint32_t N, uint32_t K, int epi, float* C, uint32_t ldc, const float* bias, bf16* Y,
-         uint32_t ldy, uint32_t splits = 1) {
+         uint32_t ldy, uint32_t splits = 1, uint64_t split_stride = 0,
+         uint32_t* splits_used = nullptr) {
     GemmArgs g;
     g.A = A;
     g.B = B;
@@ -559,8 +576,10 @@ int gemm(pp_mlp* m, const bf16* A, uint32_t lda, const bf16* B, uint32_t ldb, ui
     g.K = K;
     const uint32_t per = ((K + splits - 1) / splits + 15) & ~15u;
     g.k_split = per;
+    g.split_stride = split_stride;
     g.epi = epi;
     dim3 grid((N + 127) / 128, (M + 127) / 128, (K + per - 1) / per);
+    if (splits_used) *splits_used = grid.z;
     gemm_nt_kernel<<<grid, 256, 0, m->stream>>>(g);
     PP_HIP(hipGetLastError());
     return PP_OK;
@@ -685,15 +704,17 @@ int loss_grad(pp_mlp* m, uint32_t M, const float* d_logn, const float* d_k, cons
             dim3 g((Np + 31) / 32, (R + 31) / 32);
             transpose_kernel<<<g, 256, 0, m->stream>>>(dz, R, Np, ldz, (bf16*)m->dZT.p, R);
         }
-        // dWt_l [N][K] += dZ^T · act_l  (inner dimension = batch)
+        // dWt_l [N][K] = dZ^T · act_l  (inner dimension = batch), as split-K
+        // partials added in split order: the same gradient on every run
         float* gp = (float*)m->gpad.p;
-        zero_f32_kernel<<<grid1d((uint64_t)N * K), 256, 0, m->stream>>>(gp, (uint64_t)N * K);
-        const uint32_t splits = std::max(1u, std::min(64u, R / 256));
+        const uint64_t stride = (uint64_t)N * K;
+        const uint32_t splits = std::max(1u, std::min(kMaxSplits, R / 256));
+        uint32_t used = 0;
         if ((rc = gemm(m, (const bf16*)m->dZT.p, R, (const bf16*)m->actT[l].p, R, N, K, R,
-                       EPI_ATOMIC, gp, K, nullptr, nullptr, 0, splits)))
+                       EPI_PARTIAL, gp, K, nullptr, nullptr, 0, splits, stride, &used)))
             return rc;
-        unpack_grad_kernel<<<grid1d((uint64_t)N * m->dims[l]), 256, 0, m->stream>>>(
-            gp, N, m->dims[l], K, (float*)m->grad.p + m->off_w[l]);
+        unpack_grad_splits_kernel<<<grid1d((uint64_t)N * m->dims[l]), 256, 0, m->stream>>>(
+            gp, used, stride, N, m->dims[l], K, (float*)m->grad.p + m->off_w[l]);
         bias_grad_kernel<<<(N + 63) / 64, 256, 0, m->stream>>>(dz, M, N, ldz,
                                                               (float*)m->grad.p + m->off_b[l]);
         if (l == 0) break;
@@ -759,7 +780,7 @@ int pp_mlp_create(pp_ctx* ctx, uint32_t n_in, uint32_t width, uint32_t depth, ui
         if (hipMemset(m->wt16[l].p, 0, m->wt16[l].cap) != hipSuccess)
             return bail(fail(PP_ERR_DEVICE, "hipMemset failed"));
     }
-    if ((rc = ensure(m->gpad, gmax * 4))) return bail(rc);
+    if ((rc = ensure(m->gpad, gmax * 4 * kMaxSplits))) return bail(rc);
     // He-style uniform init, biases zero.
     std::vector<float> h(off, 0.0f);
     std::mt19937_64 rng(seed);

@@ -347,16 +347,45 @@ def train_ensemble(ctx, ts, k=4, epochs=40, batch=4096, lr=3e-3, time_weight=0.5
         except Exception as e:  # re-raised on the calling thread
             errors.append(e)
 
-    threads = [threading.Thread(target=run, args=(i,)) for i in range(k)]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    if errors:
-        for m in members:
-            m.close()
-        raise errors[0]
-    return Ensemble(members), hists
+    def train_all(idx):
+        threads = [threading.Thread(target=run, args=(i,)) for i in idx]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            for m in members:
+                m.close()
+            raise errors[0]
+
+    train_all(range(k))
+    # A member whose final Poisson loss is not finite or lies well above the
+    # others' median has diverged: it is trained again from another seed (at
+    # most twice), and dropped if it still does.
+    for attempt in (1, 2):
+        final = np.array([h[-1][0] for h in hists], dtype=np.float64)
+        med = np.median(final[np.isfinite(final)]) if np.isfinite(final).any() else 0.0
+        bad = [i for i in range(len(members))
+               if not np.isfinite(final[i]) or final[i] > med + 0.1 * abs(med)]
+        if not bad:
+            break
+        if log:
+            log("retraining diverged members %s (final losses %s)" % (bad, final[bad]))
+        for i in bad:
+            members[i].close()
+            members[i] = MLP(ctx, seed=seed * 7919 + 101 * i + 100003 * attempt)
+        train_all(bad)
+    final = np.array([h[-1][0] for h in hists], dtype=np.float64)
+    med = np.median(final[np.isfinite(final)]) if np.isfinite(final).any() else 0.0
+    keep = [i for i in range(len(members)) if np.isfinite(final[i]) and final[i] <= med + 0.1 * abs(med)]
+    for i in range(len(members)):
+        if i not in keep:
+            members[i].close()
+    if len(keep) < min(2, k):
+        for i in keep:
+            members[i].close()
+        raise RuntimeError("surrogate training diverged (final losses %s)" % final)
+    return Ensemble([members[i] for i in keep]), [hists[i] for i in keep]
 
 
 def gamma_mixture(y):

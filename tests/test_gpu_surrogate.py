@@ -50,8 +50,9 @@ def ref_forward(layers, x, requires_grad=False):
 def ref_loss(y, logn, k, times, tw):
     y = y.double()
     M = y.shape[0]
-    lmu = torch.clamp(y[:, 0] + logn, max=60.0)
-    lp = torch.exp(lmu) - k * lmu
+    l = y[:, 0] + logn
+    ex = torch.clamp(l - 30.0, min=0.0)
+    lp = torch.exp(torch.clamp(l, max=30.0)) * (1 + ex + 0.5 * ex * ex) - k * l
     logpi = torch.log_softmax(y[:, 1:4], dim=1)
     la = torch.clamp(y[:, 4:7], -4, 6)
     lb = torch.clamp(y[:, 7:10], -8, 6)
@@ -132,6 +133,33 @@ def test_loss_and_gradients_match_torch_autograd(ctx):
     mlp.close()
 
 
+def test_loss_beyond_the_log_mean_cap(ctx):
+    """Rows whose log mean exceeds the cap (30) keep a finite, bounded gradient
+    equal to the reference's (mu grows linearly there), so an overshooting step
+    cannot overflow Adam's second moment."""
+    mlp = sg.MLP(ctx, seed=6)
+    dims = [8, 256, 256, 256, 256, 10]
+    layers = unpack(mlp.get_params(), dims)
+    rng = np.random.default_rng(7)
+    M, T = 256, 4
+    x, _, k, times = batch(rng, M, T)
+    logn = np.full(M, 33.0, dtype=np.float32)  # l = y0 + 33 > 30 for every row
+    dx, dln, dk, dt = to_dev(x, logn, k, times)
+    loss, g = mlp.loss_grad(dx.data_ptr(), dln.data_ptr(), dk.data_ptr(), dt.data_ptr(), T, M, 0.0)
+    assert np.all(np.isfinite(g)) and np.isfinite(loss).all()
+    y, params = ref_forward(layers, x, requires_grad=True)
+    L, lp, _ = ref_loss(y, torch.from_numpy(logn).double(), torch.from_numpy(k).double(),
+                        torch.from_numpy(times).double(), 0.0)
+    L.backward()
+    assert loss[0] == pytest.approx(float(lp), rel=2e-2)
+    gl = torch.from_numpy(g[-(10 * 256 + 10):]).double()
+    gr = torch.cat([params[-2].grad.reshape(-1), params[-1].grad.reshape(-1)]).double()
+    cos = float(gl @ gr / (gl.norm() * gr.norm()))
+    say("capped loss lib %.4g ref %.4g, output-layer grad cos %.5f" % (loss[0], float(lp), cos))
+    assert cos > 0.99
+    mlp.close()
+
+
 def test_training_fits_a_known_yield_function(ctx):
     """Poisson rows with log-yield f(x) = -9 + 2 x0 - x1²: the Poisson NLL must
     approach its floor and the fitted log yield f within 0.1."""
@@ -204,3 +232,23 @@ def test_ensemble_gate_against_held_out_mc():
         _, _, wide = hy.predict(np.concatenate([p["raw"] for p in pairs]))
         assert np.all(a["gated"] >= wide)
         ens.close()
+
+
+def test_training_is_reproducible(ctx):
+    """Weight gradients are split-K partials added in a fixed order (no
+    atomics), so two trainings from the same seed on the same rows in the same
+    order end with bitwise equal parameters."""
+    rng = np.random.default_rng(8)
+    M = 16384
+    x = rng.uniform(-1, 1, (M, 8)).astype(np.float32)
+    logn = np.full(M, math.log(1e5), dtype=np.float32)
+    k = rng.poisson(np.exp(-6.0 + x[:, 0] + logn)).astype(np.float32)
+    times = rng.gamma(2.0, 10.0, (M, 4)).astype(np.float32)
+    ts = sg.TrainingSet(raw=x, x=x, log_n=logn, k=k, times=times)
+    params = []
+    for _ in range(2):
+        mlp = sg.MLP(ctx, seed=12)
+        sg.train(mlp, ts, epochs=2, batch=4096, lr=3e-3, time_weight=0.5, seed=5)
+        params.append(mlp.get_params())
+        mlp.close()
+    assert np.array_equal(params[0], params[1])
