@@ -20,7 +20,7 @@ using ..Emission
 using ..Detection
 
 export propagate_distance, propagate, propagate_batch, PPHit, PPSummary, HIPContext,
-       HIPServer, submit!, wait_done, take_result!
+       HIPServer, submit!, wait_done, take_result!, server_stats, PPServerStats
 
 const LIB = get(ENV, "PHOTONPROP_LIB", joinpath(@__DIR__, "..", "lib", "libphotonprop.so"))
 const C_VAC = 0.299792458  # m/ns
@@ -94,6 +94,20 @@ struct PPServerConfig
     reserved::UInt32
 end
 
+struct PPServerStats
+    submitted::UInt64
+    completed::UInt64
+    in_flight::UInt64
+    pending::UInt64
+    kernel_launches::UInt64
+    pool_records_free::UInt64
+    tiles_published::UInt64
+    grid_blocks::UInt64
+    device_stops::UInt64
+    kernel_running::UInt64
+    host_pool::UInt64
+end
+
 struct PPSummary
     n_sim::UInt64
     n_steps::UInt64
@@ -115,6 +129,7 @@ function __init__()
     @assert sizeof(PPHit) == 64 && sizeof(PPSummary) == 64 && sizeof(PPEmitter) == 48
     @assert sizeof(PPMediumTables) == 48 && sizeof(PPSpectrum) == 32
     @assert sizeof(PPRequest) == 80 && sizeof(PPServerConfig) == 32
+    @assert sizeof(PPServerStats) == 88
 end
 
 last_error() = unsafe_string(ccall((:pp_last_error, LIB), Cstring, ()))
@@ -313,6 +328,13 @@ function take_result!(srv::HIPServer, ticket::Integer; hits_cap::Integer=0)
     rc == PP_ERR_OVERFLOW || check(rc)
     resize!(hits, Int(n[]))
     return hits, summ[], lat[]
+end
+
+"""server_stats(srv) -> PPServerStats (queue, kernel and pool counters; pp_server_stats)"""
+function server_stats(srv::HIPServer)
+    st = Ref{PPServerStats}()
+    check(ccall((:pp_server_stats_get, LIB), Cint, (Ptr{Cvoid}, Ref{PPServerStats}), srv.handle, st))
+    return st[]
 end
 
 const _ctx = Dict{UInt64,HIPContext}()

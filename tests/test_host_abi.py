@@ -169,3 +169,53 @@ def test_request_dtype_layout():
     r = pp.make_requests(2)
     assert r.dtype.itemsize == 80
     assert r["max_steps"][0] == 10
+
+
+def test_julia_struct_layouts_match_header():
+    """The isbits mirrors in PhotonPropHIP.jl (not executable here) lay out
+    like the C structs: each field at its natural alignment, field for field
+    against the ctypes mirror of include/photonprop.h (whose layout the test
+    above checks in C)."""
+    jl = open(os.path.join(ROOT, "photonprop.jl_amd", "julia", "PhotonPropHIP.jl")).read()
+    prim = {"UInt64": (8, 8), "Int64": (8, 8), "Float64": (8, 8), "UInt32": (4, 4),
+            "Int32": (4, 4), "Float32": (4, 4), "Cint": (4, 4)}
+    structs = {}
+    for name, body in re.findall(r"^struct (PP\w+)\n(.*?)^end", jl, re.S | re.M):
+        structs[name] = [tuple(x.strip() for x in line.split("::")) for line in body.splitlines()
+                         if "::" in line]
+
+    def layout(name):
+        off, align, fields = 0, 1, []
+        for fname, ty in structs[name]:
+            m = re.fullmatch(r"NTuple\{(\d+),(\w+)\}", ty)
+            if m:
+                es, ea = prim[m.group(2)]
+                size, al = es * int(m.group(1)), ea
+            elif ty.startswith("Ptr{"):
+                size, al = 8, 8
+            elif ty in prim:
+                size, al = prim[ty]
+            else:
+                size, al, _ = layout(ty)
+            off = (off + al - 1) // al * al
+            fields.append((fname, off))
+            off += size
+            align = max(align, al)
+        return (off + align - 1) // align * align, align, fields
+
+    pairs = {"PPMediumTables": _abi.pp_medium_tables, "PPSpectrum": _abi.pp_spectrum,
+             "PPEmitter": _abi.pp_emitter, "PPSphere": _abi.pp_sphere, "PPHit": None,
+             "PPRequest": None, "PPServerConfig": _abi.pp_server_config,
+             "PPServerStats": _abi.pp_server_stats, "PPSummary": _abi.pp_summary}
+    assert set(pairs) <= set(structs), set(pairs) - set(structs)
+    for jname, cty in pairs.items():
+        size, _, fields = layout(jname)
+        if cty is None:  # records the Python side keeps as numpy dtypes
+            continue
+        assert size == ctypes.sizeof(cty), (jname, size, ctypes.sizeof(cty))
+        cfields = [(f[0], getattr(cty, f[0]).offset) for f in cty._fields_]
+        assert [o for _, o in fields] == [o for _, o in cfields], (jname, fields, cfields)
+    hit = layout("PPHit")
+    assert hit[0] == _abi.HIT_DTYPE.itemsize == 64
+    assert [o for _, o in hit[2]] == [_abi.HIT_DTYPE.fields[n][1] for n in _abi.HIT_DTYPE.names]
+    assert layout("PPRequest")[0] == 80
