@@ -342,7 +342,37 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter "
                          "calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--serve-probe", type=float, default=0,
+                    help="only run the 64 geometries at this many photons each through one "
+                         "batched launch and through the serving loop (all submitted at once), "
+                         "twice each, and exit (for profiling serve_kernel)")
     args = ap.parse_args()
+    if args.serve_probe:
+        import photonprop as pp
+        from photonprop import workloads
+
+        spec = pp.InverseSquareSpectrum(300.0, 600.0)
+        ctx = pp.Context(pp.SeawaterConfigMedium(), device=0)
+        reqs = workloads.scale_geometries(photons=int(args.serve_probe), seed=1)
+        batch_ms, serve_ms = [], []
+        for _ in range(2):
+            _, _, sums = ctx.propagate_batch(reqs, spec, hits_cap=1, allow_overflow=True)
+            batch_ms.append(ctx.last_kernel_ms())
+        with pp.Server(ctx, spec, max_slots=256, max_tiles_per_request=16384,
+                       hit_pool_records=1 << 20) as srv:
+            for _ in range(2):
+                t0 = time.perf_counter()
+                ts = [srv.submit(r, 0) for r in reqs]
+                for t in ts:
+                    assert srv.wait(t, 60000)
+                serve_ms.append(1e3 * (time.perf_counter() - t0))
+                got = [srv.result(t, 0)[0] for t in ts]
+                assert all(g["n_steps"] == sums["n_steps"][i] for i, g in enumerate(got))
+            grid = srv.stats()["grid_blocks"]
+        print(json.dumps({"serve_probe_photons": int(args.serve_probe), "batch_kernel_ms": batch_ms,
+                          "server_wall_ms": serve_ms, "server_grid_blocks": grid}))
+        ctx.close()
+        return
     if args.copy_probe:
         import torch
 
