@@ -424,3 +424,65 @@ def test_server_leave_room_grid_and_results(ctx):
     say("grid blocks: full %d, leave_room %d (%d CUs)" % (grids[False], grids[True], n_cu))
     assert grids[True] < grids[False]
     assert grids[True] <= 7 * n_cu
+
+
+def test_server_concurrent_submitters_and_collectors(ctx):
+    """Four host threads submit, wait for and collect their own requests on one
+    server at once (the C calls release the interpreter lock, so they overlap),
+    while a fifth polls the statistics: every request completes, each equals
+    the same request alone, and the counters add up."""
+    import threading
+
+    spec = pp.InverseSquareSpectrum(300.0, 600.0)
+    per, n_thr = 12, 4
+    reqs = workloads.batched_pairs(per * n_thr, seed=97, photons=(1e3, 3e6), distance=(0.8, 5.0))
+    caps = [int(n) for n in reqs["n_photons"]]
+    got = [None] * len(reqs)
+    errors = []
+    stop = threading.Event()
+    # the pool holds every request's records at once: a collector waiting for
+    # its last request must not wait for pool room that only its own
+    # uncollected results hold
+    pool = 1 << 26
+    assert sum(caps) <= pool
+    with pp.Server(ctx, spec, max_slots=8, max_tiles_per_request=512,
+                   hit_pool_records=pool) as srv:
+        def worker(w):
+            try:
+                mine = list(range(w, len(reqs), n_thr))
+                tickets = [(i, srv.submit(reqs[i], caps[i])) for i in mine]
+                for i, t in reversed(tickets):  # collect in the opposite order
+                    assert srv.wait(t, 60000), i
+                    got[i] = srv.result(t, caps[i])
+            except Exception as e:  # reported on the main thread
+                errors.append(e)
+
+        def poller():
+            try:
+                while not stop.is_set():
+                    st = srv.stats()
+                    assert st["completed"] <= st["submitted"], st
+                    time.sleep(0.001)
+            except Exception as e:
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(w,)) for w in range(n_thr)]
+        p = threading.Thread(target=poller)
+        p.start()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        stop.set()
+        p.join()
+        st = srv.stats()
+    assert not errors, errors
+    say("concurrent submitters: stats", st)
+    assert st["submitted"] == st["completed"] == len(reqs) and st["in_flight"] == 0
+    assert st["pool_records_free"] == pool
+    for i, r in enumerate(reqs):
+        s, h, _ = got[i]
+        ref = alone(ctx, r)
+        for k in SUM_KEYS:
+            assert s[k] == ref.summary[k], (i, k)
+        np.testing.assert_array_equal(key(h), key(ref.hits))
