@@ -238,7 +238,7 @@ def bench_serving(pp, ctx, spectrum, n_req):
 
 
 def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log_std=0.05,
-                    n_eval=1024, eval_photons=100_000_000):
+                    n_eval=1024, eval_photons=100_000_000, activation=None):
     """Train the BF16-MFMA surrogate (an ensemble of `members` networks on bootstrap
     resamples of the MC rows from the batched kernel), then evaluate it on
     held-out pairs against high-statistics MC: yield errors over every pair with
@@ -265,9 +265,10 @@ def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log
           flush=True)
     t0 = time.perf_counter()
     ens, hists = sg.train_ensemble(ctx, ts, k=members, epochs=epochs, batch=4096, lr=3e-3,
-                                   time_weight=0.5, seed=0)
+                                   time_weight=0.5, seed=0, activation=activation)
     out["train_s"] = time.perf_counter() - t0
     out["members"] = members
+    out["activation"] = ens.members[0].activation
     out["epochs"] = epochs
     out["final_loss"] = [h[-1] for h in hists]
     print("surrogate: %d networks trained in %.1f s" % (members, out["train_s"]), file=sys.stderr,
@@ -336,6 +337,9 @@ def main():
     ap.add_argument("--surrogate-epochs", type=int, default=40)
     ap.add_argument("--surrogate-members", type=int, default=4,
                     help="networks in the ensemble whose spread is the predictive uncertainty")
+    ap.add_argument("--surrogate-activation", choices=("relu", "silu"), default=None,
+                    help="hidden activation of the surrogate networks (default: the library's "
+                         "Python default)")
     ap.add_argument("--mlp-probe", type=float, default=0,
                     help="only time the surrogate's fused forward on this many random rows "
                          "(random weights; for profiling) and exit")
@@ -392,13 +396,13 @@ def main():
 
         n = int(args.mlp_probe)
         ctx = pp.Context(pp.SeawaterConfigMedium(), device=0)
-        mlp = sg.MLP(ctx, seed=0)
+        mlp = sg.MLP(ctx, seed=0, activation=args.surrogate_activation)
         xq = torch.rand((n, 8), device="cuda:0") * 2 - 1
         yq = torch.empty((n, 10), device="cuda:0")
         torch.cuda.synchronize()  # the library's stream does not wait for torch's
         times = [mlp.forward_device(xq.data_ptr(), yq.data_ptr(), n) for _ in range(5)]
         fl = n * 2.0 * (8 * 256 + 3 * 256 * 256 + 256 * 10)
-        print(json.dumps({"mlp_probe_rows": n, "forward_ms": times,
+        print(json.dumps({"mlp_probe_rows": n, "activation": mlp.activation, "forward_ms": times,
                           "queries_per_s": n / (1e-3 * min(times)),
                           "tflops": fl / (1e-3 * min(times)) / 1e12}))
         mlp.close()
@@ -509,7 +513,8 @@ def main():
         if args.surrogate_rows > 0:
             out["surrogate"] = bench_surrogate(pp, ctx, int(args.surrogate_rows),
                                                int(args.surrogate_photons), args.surrogate_epochs,
-                                               members=args.surrogate_members)
+                                               members=args.surrogate_members,
+                                               activation=args.surrogate_activation)
             print("surrogate done", file=sys.stderr, flush=True)
     if args.cpu_sample > 0:
         cb = cpu_baseline(pp, int(args.cpu_sample))

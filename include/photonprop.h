@@ -325,8 +325,8 @@ int pp_server_destroy(pp_server* srv);
 
 /* ---- MLP surrogate (BF16 MFMA training and inference) ---------------------- */
 /* Role of PhotonProp.jl's Modelling module (src/modelling.jl:52-111, :202-267):
- * an MLP n_in -> width^depth -> n_out with SiLU hidden activations, BF16
- * weights and activations, FP32 accumulation, FP32 master weights and Adam.
+ * an MLP n_in -> width^depth -> n_out with SiLU or ReLU hidden activations,
+ * BF16 weights and activations, FP32 accumulation, FP32 master weights and Adam.
  * The training loss is the Poisson NLL of the observed (weighted) yield k with
  * expectation N·exp(y0) plus time_weight times the NLL of residual arrival
  * times (ns) under a 3-component gamma mixture parametrised by y[1..3]
@@ -341,6 +341,15 @@ int pp_mlp_destroy(pp_mlp* mlp);
 int pp_mlp_n_params(pp_mlp* mlp, uint64_t* n);
 int pp_mlp_get_params(pp_mlp* mlp, float* params, uint64_t n);
 int pp_mlp_set_params(pp_mlp* mlp, const float* params, uint64_t n);
+
+/* Hidden-layer activation.  The reference's network is ReLU throughout
+ * (src/modelling.jl:212-220, Dense(..., relu)); SiLU is smooth in the inputs. */
+#define PP_MLP_SILU 0u
+#define PP_MLP_RELU 1u
+/* Select the hidden activation (PP_MLP_SILU after pp_mlp_create).  It applies
+ * to every later forward, loss/gradient and training call; the parameters and
+ * the Adam state are kept. */
+int pp_mlp_set_activation(pp_mlp* mlp, uint32_t activation);
 
 /* y[n_rows][n_out] = MLP(x[n_rows][n_in]); device pointers, FP32 in and out. */
 int pp_mlp_forward_device(pp_mlp* mlp, const float* d_x, uint64_t n_rows, float* d_y);

@@ -43,8 +43,8 @@ inline uint32_t pad16(uint32_t x) { return (x + 15u) & ~15u; }
 enum Epilogue : int {
     EPI_STORE = 0,      // C = acc (fp32)
     EPI_PARTIAL = 1,    // C[split] = acc (fp32; split-K partials, summed in order afterwards)
-    EPI_BIAS_SILU = 2,  // C = z = acc + bias (fp32), Y = silu(z) (bf16)
-    EPI_SILU_ONLY = 3,  // Y = silu(acc + bias) (bf16), C untouched
+    EPI_BIAS_ACT = 2,   // C = z = acc + bias (fp32), Y = act(z) (bf16)
+    EPI_ACT_ONLY = 3,   // Y = act(acc + bias) (bf16), C untouched
     EPI_BIAS = 4,       // C = acc + bias (fp32)
 };
 
@@ -58,12 +58,17 @@ struct GemmArgs {
     uint32_t M, N, K, k_split;
     uint64_t split_stride;  // EPI_PARTIAL: floats between the partials of consecutive splits
     int epi;
+    uint32_t act;  // PP_MLP_SILU or PP_MLP_RELU (EPI_BIAS_ACT, EPI_ACT_ONLY)
 };
 
 // z·sigmoid(z); the 1-ulp hardware reciprocal instead of an IEEE division (the
 // result is rounded to bf16 or feeds a bf16 product).
 __device__ __forceinline__ float silu(float z) {
     return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+__device__ __forceinline__ float hidden_act(float z, uint32_t act) {
+    return act == PP_MLP_RELU ? fmaxf(z, 0.0f) : silu(z);
 }
 
 // 128x128 block tile, 4 waves of 64x64 (2x2 MFMA 32x32x16 tiles each).
@@ -107,7 +112,7 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
         for (int b = 0; b < 2; ++b) {
             const uint32_t j = j0 + b * 32u + r;
             if (j >= g.N) continue;
-            const float bj = (g.epi >= EPI_BIAS_SILU) ? g.bias[j] : 0.0f;
+            const float bj = (g.epi >= EPI_BIAS_ACT) ? g.bias[j] : 0.0f;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const uint32_t i = i0 + a * 32u + 8u * (q >> 2) + 4u * (lane >> 5) + (q & 3);
@@ -118,13 +123,15 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
                 case EPI_PARTIAL:
                     g.C[(size_t)blockIdx.z * g.split_stride + (size_t)i * g.ldc + j] = v;
                     break;
-                case EPI_BIAS_SILU: {
+                case EPI_BIAS_ACT: {
                     const float z = v + bj;
                     g.C[(size_t)i * g.ldc + j] = z;
-                    g.Y[(size_t)i * g.ldy + j] = (bf16)silu(z);
+                    g.Y[(size_t)i * g.ldy + j] = (bf16)hidden_act(z, g.act);
                     break;
                 }
-                case EPI_SILU_ONLY: g.Y[(size_t)i * g.ldy + j] = (bf16)silu(v + bj); break;
+                case EPI_ACT_ONLY:
+                    g.Y[(size_t)i * g.ldy + j] = (bf16)hidden_act(v + bj, g.act);
+                    break;
                 default: g.C[(size_t)i * g.ldc + j] = v + bj; break;
                 }
             }
@@ -134,18 +141,20 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
 // Fused inference: every layer of the MLP for a tile of kFusedRows rows in
 // one block, the activations resident in LDS as bf16 rows (input features,
 // then each hidden layer's output written over its input).  Each layer is
-// D = Wt_l · Aᵀ on v_mfma_f32_32x32x16_bf16: the weights (rows = output
-// features, contiguous along k) are the A operand, read from global memory
-// (L2-resident: every block reads the same weights) kFusedAhead k-steps ahead
-// of use, the activations the B operand, read from LDS.  Wave w owns output
-// features [32w, 32w + 32) for all 128 rows (four 32x32 accumulator tiles).
-// (Two feature tiles per wave halve the LDS reads but need 235 VGPRs, two
-// waves per SIMD, and measured 20 % slower than eight waves at 128 VGPRs.)
-// In D's layout a lane holds one row of the tile (lane % 32) and four
-// consecutive output features per group of four accumulator registers, so the
-// epilogue (bias, SiLU, bf16) writes 8 bytes per group back to LDS.  Only the
-// fp32 inputs and outputs touch HBM.  Limits: every padded width <=
+// D = Wt_l · Aᵀ on v_mfma_f32_32x32x16_bf16: the weights are the A operand,
+// read from global memory in fragment order (L2-resident: every block reads
+// the same weights) kFusedAhead k-steps ahead of use, the activations the B
+// operand, read from LDS one k-step ahead.  Wave w owns output features
+// [32w, 32w + 32) for all 128 rows (four 32x32 accumulator tiles).  In D's
+// layout a lane holds one row of the tile (lane % 32) and four consecutive
+// output features per group of four accumulator registers, so the epilogue
+// (bias, activation, bf16) writes 8 bytes per group back to LDS.  The output
+// layer (<= 16 features) runs on 16x16x32 MFMAs, one 16-row tile per wave.
+// Only the fp32 inputs and outputs touch HBM.  Limits: every padded width <=
 // kFusedMaxWidth.  Two blocks share a CU (LDS 2 x 66 KiB, 4 waves per SIMD).
+// Measured and dropped: 64x64 tiles per wave (each LDS fragment feeding two
+// MFMAs, each weight fragment read by two waves) ran at half the speed, as
+// the L2 requests doubled; the L1 does not merge the second wave's reads.
 constexpr uint32_t kFusedRows = 128;      // 4 sample tiles of 32
 constexpr uint32_t kFusedMaxWidth = 256;  // 8 feature tiles: one per wave
 constexpr uint32_t kFusedLd = kFusedMaxWidth + 8;  // LDS row stride (bf16): rows 16 B apart
@@ -158,48 +167,186 @@ struct FusedArgs {
     uint32_t M, n_in, ld_x, n_out, ld_out, n_layers;
     uint32_t dims[kMaxLayers + 1];
     uint32_t pdim[kMaxLayers + 1];
-    const bf16* wt[kMaxLayers];
+    const bf16x8* wfrag[kMaxLayers];  // weights in fragment order (frag_geom)
     const float* bias[kMaxLayers];
 };
 
+// The fused kernel reads each layer's weights in MFMA fragment order, so that
+// one wave's load of a fragment is 1 KiB of contiguous memory (8 full cache
+// lines): wfrag[(tile·steps + s)·64 + lane] holds the 8 bf16
+// Wt[tile·rows + lane % rows][s·kstep + 8·(lane / rows) + 0..7], zero past the
+// padded matrix.  Hidden layers feed v_mfma_f32_32x32x16 (tiles of 32 output
+// features, k-steps of 16), the output layer v_mfma_f32_16x16x32 (16 features,
+// k-steps of 32).  Read straight from the row-major Wt, a fragment is 32 B of
+// 32 different rows: 32 cache lines for every 1 KiB, which the L1 could not
+// keep for the 16 waves of a CU, so the L2 delivered every line four times.
+struct FragGeom {
+    uint32_t rows, kstep, tiles, steps;
+};
+__host__ __device__ inline FragGeom frag_geom(uint32_t no_pad, uint32_t k_pad, bool output_layer) {
+    FragGeom g;
+    g.rows = output_layer ? 16u : 32u;
+    g.kstep = output_layer ? 32u : 16u;
+    g.tiles = (no_pad + g.rows - 1) / g.rows;
+    g.steps = (k_pad + g.kstep - 1) / g.kstep;
+    return g;
+}
+
 // One layer's k-loop for a width known at compile time (kSteps k-steps of 16),
-// fully unrolled: the weight fragments are prefetched kFusedAhead steps ahead
-// and the compiler waits for exactly the one each step needs (a rolled loop
-// waited for all of them at its back edge), and a step's four activation
-// fragments are read before its four MFMAs.  The loads are unconditional (a
-// lane past the layer's features reads row 0: its accumulator rows are output
-// features the epilogue does not keep), so the code is straight-line.  About
-// 2 % of the kernel's time; the epilogue below gave the rest of the gain.
+// fully unrolled: the weight fragments are prefetched kFusedAhead steps ahead,
+// and the next step's four activation fragments are read from LDS before this
+// step's four MFMAs are issued, so the LDS latency runs under the MFMAs (left
+// to itself the compiler read each fragment right before its MFMA and waited
+// for it there).  The loads are unconditional (a lane past the layer's
+// features reads row 0: its accumulator rows are output features the epilogue
+// does not keep), so the code is straight-line.
 template <int kSteps>
-__device__ __forceinline__ void fused_mfma_steps(const bf16* wrow, const bf16* act, uint32_t li,
+__device__ __forceinline__ void fused_mfma_steps(const bf16x8* wfr, const bf16* act, uint32_t li,
                                                  uint32_t kh, f32x16 (&acc)[4]) {
     bf16x8 wf[kFusedAhead];
 #pragma unroll
-    for (int u = 0; u < kFusedAhead && u < kSteps; ++u)
-        wf[u] = *reinterpret_cast<const bf16x8*>(wrow + u * 16);
+    for (int u = 0; u < kFusedAhead && u < kSteps; ++u) wf[u] = wfr[u * 64];
+    const bf16* arow = act + li * kFusedLd + kh;
+    bf16x8 xb[2][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        xb[0][t] = *reinterpret_cast<const bf16x8*>(arow + t * 32u * kFusedLd);
 #pragma unroll
     for (int st = 0; st < kSteps; ++st) {
         const bf16x8 w = wf[st % kFusedAhead];
-        if (st + kFusedAhead < kSteps)
-            wf[st % kFusedAhead] = *reinterpret_cast<const bf16x8*>(wrow + (st + kFusedAhead) * 16);
-        // keep the prefetch here, kFusedAhead steps ahead of its use: left to
-        // itself the scheduler sinks each load next to its MFMAs
+        if (st + kFusedAhead < kSteps) wf[st % kFusedAhead] = wfr[(st + kFusedAhead) * 64];
+        if (st + 1 < kSteps) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                xb[(st + 1) & 1][t] = *reinterpret_cast<const bf16x8*>(
+                    arow + t * 32u * kFusedLd + (st + 1) * 16u);
+        }
+        // keep both prefetches ahead of this step's MFMAs
         __builtin_amdgcn_sched_barrier(0);
-        bf16x8 xb[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-            xb[t] = *reinterpret_cast<const bf16x8*>(act + (t * 32u + li) * kFusedLd + st * 16u + kh);
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb[t], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb[st & 1][t], acc[t], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+static_assert(kFusedRows == 16 * (kFusedThreads / 64), "one 16-row output tile per wave");
+
+// The output layer (at most 16 features, pdim 16) on v_mfma_f32_16x16x32_bf16,
+// one 16-row tile per wave, so that all eight waves share it (as one 32-wide
+// feature tile it ran on a single wave, as long as a whole hidden layer).
+// D[feature][row] = Wt[feature][k] · act[row][k]; lane l holds A row l%16 and
+// B column l%16 with k = 8·(l/16) + 0..7 of each 32-wide k-step, and D rows
+// 4·(l/16) + 0..3 of column l%16.  K is a multiple of 16; a k-group past it
+// is zero.
+__device__ __forceinline__ void fused_output_layer(const FusedArgs& f, uint32_t l,
+                                                   const bf16* act, uint32_t wave, uint32_t lane,
+                                                   uint32_t row0) {
+    const uint32_t K = f.pdim[l];
+    const uint32_t c = lane & 15u, kg = (lane >> 4) * 8u;
+    const uint32_t rtile = wave * 16u;
+    const bf16x8* wfr = f.wfrag[l] + lane;  // one 16-feature tile
+    const bf16* arow = act + (rtile + c) * kFusedLd + kg;
+    const bf16x8 zero = {};
+    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0, s = 0; k < K; k += 32u, ++s) {
+        // the weights are zero past K; the LDS row there is not
+        const bf16x8 b = k + kg < K ? *reinterpret_cast<const bf16x8*>(arow + k) : zero;
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfr[s * 64u], b, d, 0, 0, 0);
+    }
+    const uint32_t i = row0 + rtile + c;
+    if (i < f.M) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t j = (lane >> 4) * 4u + q;
+            if (j < f.n_out) f.out[(size_t)i * f.ld_out + j] = d[q] + f.bias[l][j];
+        }
+    }
+}
+
+// One hidden layer of the fused kernel: wave w owns feature tile w for all
+// 128 rows (acc[t] = row tile t); products, then (after every wave has read
+// the layer's input) bias, activation and bf16 written over the input.
+template <bool kRelu>
+__device__ __forceinline__ void fused_hidden_layer(const FusedArgs& f, uint32_t l, bf16* act,
+                                                   uint32_t wave, uint32_t lane) {
+    const uint32_t li = lane & 31u, kh = (lane >> 5) * 8u;
+    const uint32_t K = f.pdim[l], NO = f.pdim[l + 1], n_true = f.dims[l + 1];
+    const bool mine = wave * 32u < NO;  // wave-uniform: this wave has a feature tile
+    // this wave's feature tile; rows past NO are zero
+    const bf16x8* wfr = f.wfrag[l] + (mine ? (size_t)wave * (K / 16u) * 64u : 0u) + lane;
+    f32x16 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[t][q] = 0.0f;
+    if (mine && K == kFusedMaxWidth) {
+        fused_mfma_steps<kFusedMaxWidth / 16>(wfr, act, li, kh, acc);
+    } else if (mine) {
+        const bf16x8 zero = {};
+        bf16x8 wf[kFusedAhead];
+#pragma unroll
+        for (int u = 0; u < kFusedAhead; ++u) wf[u] = (uint32_t)u * 16u < K ? wfr[u * 64] : zero;
+        for (uint32_t k = 0; k < K; k += 16u * kFusedAhead) {
+#pragma unroll
+            for (int u = 0; u < kFusedAhead; ++u) {
+                const uint32_t kk = k + 16u * u;
+                if (kk >= K) break;
+                const bf16x8 w = wf[u];
+                const uint32_t kn = kk + 16u * kFusedAhead;
+                if (kn < K) wf[u] = wfr[(kn / 16u) * 64u];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const bf16x8 xb = *reinterpret_cast<const bf16x8*>(
+                        act + (t * 32u + li) * kFusedLd + kk + kh);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    __syncthreads();  // every wave has read this layer's input
+    if (mine) {
+        const float* bias = f.bias[l];
+        // this lane's 16 biases (D rows 8g + 4*(lane/32) + r), loaded once per
+        // layer rather than once per row tile, and the activation computed for
+        // every lane and then selected: the guarded per-element loads and
+        // branches this replaces took a fifth of the kernel's time
+        float bj[4][4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const uint32_t jg = wave * 32u + 8u * g + 4u * (lane >> 5);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bj[g][r] = jg + r < n_true ? bias[jg + r] : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t i = t * 32u + li;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                // D rows 8g + 4*(lane/32) + 0..3 of this wave's feature tile
+                const uint32_t jg = wave * 32u + 8u * g + 4u * (lane >> 5);
+                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                bf16x4 y;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float v = acc[t][4 * g + r] + bj[g][r];
+                    // computed for every lane, then selected: no branch
+                    const float sv = kRelu ? fmaxf(v, 0.0f) : silu(v);
+                    y[r] = (bf16)(jg + r < n_true ? sv : 0.0f);
+                }
+                *reinterpret_cast<bf16x4*>(act + i * kFusedLd + jg) = y;
+            }
+        }
+    }
+    __syncthreads();  // this layer's output is the next layer's input
+}
+
+template <bool kRelu>
 __global__ void __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(4)))
 mlp_forward_fused_kernel(FusedArgs f) {
     extern __shared__ bf16 act[];  // [kFusedRows][kFusedLd]
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-    const uint32_t li = lane & 31u, kh = (lane >> 5) * 8u;
     const uint32_t row0 = blockIdx.x * kFusedRows;
     // input tile: fp32 -> bf16, zero beyond n_in and beyond M
     const uint32_t K0 = f.pdim[0];
@@ -210,88 +357,8 @@ mlp_forward_fused_kernel(FusedArgs f) {
         act[i * kFusedLd + k] = (bf16)v;
     }
     __syncthreads();
-    const bf16x8 zero = {};
-    for (uint32_t l = 0; l < f.n_layers; ++l) {
-        const uint32_t K = f.pdim[l], NO = f.pdim[l + 1], n_true = f.dims[l + 1];
-        const bool last = l + 1 == f.n_layers;
-        const bool mine = wave * 32u < NO;  // wave-uniform: this wave has a feature tile
-        const uint32_t j = wave * 32u + li;
-        const bool jok = mine && j < NO;
-        const bf16* wrow = f.wt[l] + (size_t)(jok ? j : 0u) * K + kh;
-        f32x16 acc[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[t][q] = 0.0f;
-        if (mine && K == kFusedMaxWidth) {
-            fused_mfma_steps<kFusedMaxWidth / 16>(wrow, act, li, kh, acc);
-        } else if (mine) {
-            bf16x8 wf[kFusedAhead];
-#pragma unroll
-            for (int u = 0; u < kFusedAhead; ++u)
-                wf[u] = (jok && (uint32_t)u * 16u < K)
-                            ? *reinterpret_cast<const bf16x8*>(wrow + u * 16u) : zero;
-            for (uint32_t k = 0; k < K; k += 16u * kFusedAhead) {
-#pragma unroll
-                for (int u = 0; u < kFusedAhead; ++u) {
-                    const uint32_t kk = k + 16u * u;
-                    if (kk >= K) break;
-                    const bf16x8 w = wf[u];
-                    const uint32_t kn = kk + 16u * kFusedAhead;
-                    if (jok && kn < K) wf[u] = *reinterpret_cast<const bf16x8*>(wrow + kn);
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const bf16x8 xb = *reinterpret_cast<const bf16x8*>(
-                            act + (t * 32u + li) * kFusedLd + kk + kh);
-                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, acc[t], 0, 0, 0);
-                    }
-                }
-            }
-        }
-        __syncthreads();  // every wave has read this layer's input
-        if (mine) {
-            const float* bias = f.bias[l];
-            // this lane's 16 biases (D rows 8g + 4*(lane/32) + r), loaded once
-            // per layer rather than once per row tile, and SiLU computed for
-            // every lane and then selected: the guarded per-element loads and
-            // branches this replaces took a fifth of the kernel's time
-            float bj[4][4];
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const uint32_t jg = wave * 32u + 8u * g + 4u * (lane >> 5);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) bj[g][r] = jg + r < n_true ? bias[jg + r] : 0.0f;
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const uint32_t i = t * 32u + li;
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    // D rows 8g + 4*(lane/32) + 0..3 of this wave's feature tile
-                    const uint32_t jg = wave * 32u + 8u * g + 4u * (lane >> 5);
-                    float v[4];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = acc[t][4 * g + r] + bj[g][r];
-                    if (!last) {
-                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                        bf16x4 y;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            // computed for every lane, then selected: no branch
-                            const float sv = silu(v[r]);
-                            y[r] = (bf16)(jg + r < n_true ? sv : 0.0f);
-                        }
-                        *reinterpret_cast<bf16x4*>(act + i * kFusedLd + jg) = y;
-                    } else if (row0 + i < f.M) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (jg + r < f.n_out) f.out[(size_t)(row0 + i) * f.ld_out + jg + r] = v[r];
-                    }
-                }
-            }
-        }
-        __syncthreads();  // this layer's output is the next layer's input
-    }
+    for (uint32_t l = 0; l + 1 < f.n_layers; ++l) fused_hidden_layer<kRelu>(f, l, act, wave, lane);
+    fused_output_layer(f, f.n_layers - 1, act, wave, lane, row0);
 }
 
 // out[c][r] = in[r][c] for an R x C bf16 matrix (ld_in >= C, ld_out >= R).
@@ -323,9 +390,10 @@ __global__ void pack_input_kernel(const float* x, uint32_t M, uint32_t M_pad, ui
     }
 }
 
-// dZ = dA ⊙ silu'(z), bf16 out (rows >= M zeroed up to M_pad).
-__global__ void silu_backward_kernel(const float* dA, const float* Z, uint32_t M, uint32_t M_pad,
-                                     uint32_t N, uint32_t ld, bf16* dZ) {
+// dZ = dA ⊙ act'(z), bf16 out (rows >= M zeroed up to M_pad).  ReLU's
+// derivative at z = 0 is taken as 0, as in Flux/Zygote.
+__global__ void act_backward_kernel(const float* dA, const float* Z, uint32_t M, uint32_t M_pad,
+                                    uint32_t N, uint32_t ld, uint32_t act, bf16* dZ) {
     const uint64_t n = (uint64_t)M_pad * N;
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -333,8 +401,12 @@ __global__ void silu_backward_kernel(const float* dA, const float* Z, uint32_t M
         float d = 0.0f;
         if (i < M) {
             const float z = Z[(size_t)i * ld + j];
-            const float s = 1.0f / (1.0f + __expf(-z));
-            d = dA[(size_t)i * ld + j] * s * (1.0f + z * (1.0f - s));
+            if (act == PP_MLP_RELU) {
+                d = z > 0.0f ? dA[(size_t)i * ld + j] : 0.0f;
+            } else {
+                const float s = 1.0f / (1.0f + __expf(-z));
+                d = dA[(size_t)i * ld + j] * s * (1.0f + z * (1.0f - s));
+            }
         }
         dZ[(size_t)i * ld + j] = (bf16)d;
     }
@@ -480,6 +552,27 @@ __global__ void sync_weights_kernel(const float* wt, uint32_t n_out, uint32_t n_
     }
 }
 
+// Layer weights in the fused kernel's fragment order (see frag_geom) from the
+// padded row-major bf16 copy wt16 [no_pad][k_pad].
+__global__ void pack_fragments_kernel(const bf16* wt16, uint32_t no_pad, uint32_t k_pad,
+                                      FragGeom g, bf16x8* out) {
+    const uint64_t n = (uint64_t)g.tiles * g.steps * 64u;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t lane = (uint32_t)(t & 63u);
+        const uint32_t s = (uint32_t)((t >> 6) % g.steps), tile = (uint32_t)((t >> 6) / g.steps);
+        const uint32_t r = tile * g.rows + lane % g.rows;
+        const uint32_t k0 = s * g.kstep + 8u * (lane / g.rows);
+        bf16x8 v = {};
+        if (r < no_pad) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (k0 + i < k_pad) v[i] = wt16[(size_t)r * k_pad + k0 + i];
+        }
+        out[t] = v;
+    }
+}
+
 // Gradient of layer weights in [n_out][ldk] padded form -> packed [n_out][n_in].
 // The weight gradient from its split-K partials, added in split order (the
 // same sum on every run, unlike atomics), unpadded into the packed vector.
@@ -521,11 +614,13 @@ struct pp_mlp {
     DevBuf w, grad, adam_m, adam_v;     // fp32 master / grads / Adam state (packed)
     DevBuf gpad;                        // fp32 padded weight-gradient scratch
     DevBuf wt16[kMaxLayers], w16[kMaxLayers];
+    DevBuf wfrag[kMaxLayers];           // bf16 weights in fragment order (fused inference)
     // activations (grow with batch)
     uint32_t cap_rows = 0;
     DevBuf x16, act[kMaxLayers + 1], actT[kMaxLayers + 1], z[kMaxLayers + 1], dA, dZ, dZT, dY,
         yout, loss;
     uint64_t step = 0;
+    uint32_t activation = PP_MLP_SILU;  // hidden activation
 };
 
 namespace {
@@ -578,6 +673,7 @@ int gemm(pp_mlp* m, const bf16* A, uint32_t lda, const bf16* B, uint32_t ldb, ui
     g.k_split = per;
     g.split_stride = split_stride;
     g.epi = epi;
+    g.act = m->activation;
     dim3 grid((N + 127) / 128, (M + 127) / 128, (K + per - 1) / per);
     if (splits_used) *splits_used = grid.z;
     gemm_nt_kernel<<<grid, 256, 0, m->stream>>>(g);
@@ -592,6 +688,9 @@ int sync_weights(pp_mlp* m) {
         sync_weights_kernel<<<grid1d((uint64_t)ldn * ldk), 256, 0, m->stream>>>(
             (const float*)m->w.p + m->off_w[l], no, ni, (bf16*)m->wt16[l].p, ldk,
             (bf16*)m->w16[l].p, ldn, ldk, ldn);
+        const FragGeom g = frag_geom(ldn, ldk, l + 1 == m->n_layers);
+        pack_fragments_kernel<<<grid1d((uint64_t)g.tiles * g.steps * 64u), 256, 0, m->stream>>>(
+            (const bf16*)m->wt16[l].p, ldn, ldk, g, (bf16x8*)m->wfrag[l].p);
     }
     PP_HIP(hipGetLastError());
     return PP_OK;
@@ -608,10 +707,10 @@ int forward(pp_mlp* m, uint32_t M, bool train, float* out, uint32_t ld_out) {
         if (l + 1 < L) {
             bf16* y = (bf16*)m->act[l + 1].p;
             if (train)
-                rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_BIAS_SILU,
+                rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_BIAS_ACT,
                           (float*)m->z[l + 1].p, m->pdim[l + 1], b, y, m->pdim[l + 1]);
             else
-                rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_SILU_ONLY, nullptr, 0,
+                rc = gemm(m, a, K, (const bf16*)m->wt16[l].p, K, M, N, K, EPI_ACT_ONLY, nullptr, 0,
                           b, y, m->pdim[l + 1]);
             if (rc) return rc;
             a = y;
@@ -649,15 +748,20 @@ int forward_fused(pp_mlp* m, const float* d_x, uint32_t M, float* d_y) {
         fa.pdim[l] = m->pdim[l];
     }
     for (uint32_t l = 0; l < L; ++l) {
-        fa.wt[l] = (const bf16*)m->wt16[l].p;
+        fa.wfrag[l] = (const bf16x8*)m->wfrag[l].p;
         fa.bias[l] = (const float*)m->w.p + m->off_b[l];
     }
     const size_t lds = (size_t)kFusedRows * kFusedLd * sizeof(bf16);
-    PP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(mlp_forward_fused_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const bool relu = m->activation == PP_MLP_RELU;
+    const void* fn = relu ? reinterpret_cast<const void*>(mlp_forward_fused_kernel<true>)
+                          : reinterpret_cast<const void*>(mlp_forward_fused_kernel<false>);
+    PP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const uint32_t grid = (M + kFusedRows - 1) / kFusedRows;
     PP_HIP(hipEventRecord(m->ev0, m->stream));
-    mlp_forward_fused_kernel<<<grid, kFusedThreads, lds, m->stream>>>(fa);
+    if (relu)
+        mlp_forward_fused_kernel<true><<<grid, kFusedThreads, lds, m->stream>>>(fa);
+    else
+        mlp_forward_fused_kernel<false><<<grid, kFusedThreads, lds, m->stream>>>(fa);
     PP_HIP(hipGetLastError());
     PP_HIP(hipEventRecord(m->ev1, m->stream));
     PP_HIP(hipStreamSynchronize(m->stream));
@@ -722,8 +826,8 @@ int loss_grad(pp_mlp* m, uint32_t M, const float* d_logn, const float* d_k, cons
         if ((rc = gemm(m, dz, ldz, (const bf16*)m->w16[l].p, Np, M, K, Np, EPI_STORE,
                        (float*)m->dA.p, K, nullptr, nullptr, 0)))
             return rc;
-        silu_backward_kernel<<<grid1d((uint64_t)R * K), 256, 0, m->stream>>>(
-            (const float*)m->dA.p, (const float*)m->z[l].p, M, R, K, K, (bf16*)m->dZ.p);
+        act_backward_kernel<<<grid1d((uint64_t)R * K), 256, 0, m->stream>>>(
+            (const float*)m->dA.p, (const float*)m->z[l].p, M, R, K, K, m->activation, (bf16*)m->dZ.p);
         dz = (const bf16*)m->dZ.p;
         ldz = K;
     }
@@ -774,8 +878,10 @@ int pp_mlp_create(pp_ctx* ctx, uint32_t n_in, uint32_t width, uint32_t depth, ui
     uint64_t gmax = 0;
     for (uint32_t l = 0; l < m->n_layers; ++l) {
         gmax = std::max<uint64_t>(gmax, (uint64_t)m->dims[l + 1] * m->pdim[l]);
+        const FragGeom g = frag_geom(m->pdim[l + 1], m->pdim[l], l + 1 == m->n_layers);
         if ((rc = ensure(m->wt16[l], (size_t)m->pdim[l + 1] * m->pdim[l] * sizeof(bf16))) ||
-            (rc = ensure(m->w16[l], (size_t)m->pdim[l] * m->pdim[l + 1] * sizeof(bf16))))
+            (rc = ensure(m->w16[l], (size_t)m->pdim[l] * m->pdim[l + 1] * sizeof(bf16))) ||
+            (rc = ensure(m->wfrag[l], (size_t)g.tiles * g.steps * 64u * sizeof(bf16x8))))
             return bail(rc);
         if (hipMemset(m->wt16[l].p, 0, m->wt16[l].cap) != hipSuccess)
             return bail(fail(PP_ERR_DEVICE, "hipMemset failed"));
@@ -811,6 +917,7 @@ int pp_mlp_destroy(pp_mlp* m) {
         if (l < kMaxLayers) {
             if (m->wt16[l].p) (void)hipFree(m->wt16[l].p);
             if (m->w16[l].p) (void)hipFree(m->w16[l].p);
+            if (m->wfrag[l].p) (void)hipFree(m->wfrag[l].p);
         }
         if (m->act[l].p) (void)hipFree(m->act[l].p);
         if (m->actT[l].p) (void)hipFree(m->actT[l].p);
@@ -820,6 +927,14 @@ int pp_mlp_destroy(pp_mlp* m) {
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
+    return PP_OK;
+}
+
+int pp_mlp_set_activation(pp_mlp* m, uint32_t activation) {
+    if (!m) return fail(PP_ERR_INVALID_ARG, "mlp is NULL");
+    if (activation != PP_MLP_SILU && activation != PP_MLP_RELU)
+        return fail(PP_ERR_INVALID_ARG, "unknown activation %u", activation);
+    m->activation = activation;
     return PP_OK;
 }
 

@@ -28,6 +28,9 @@ PP_EMIT_CHERENKOV_CONE = 0
 PP_EMIT_ISOTROPIC = 1
 PP_EMIT_REF_ISOTROPIC = 2
 
+PP_MLP_SILU = 0
+PP_MLP_RELU = 1
+
 PP_MAX_TABLE_POINTS = 2048
 PP_MAX_KNOTS = 1024
 PP_MAX_STEPS = 1 << 24
@@ -188,6 +191,7 @@ SIGNATURES = {
     "pp_mlp_n_params": (ctypes.c_int, [_vp, ctypes.POINTER(_u64)]),
     "pp_mlp_get_params": (ctypes.c_int, [_vp, _vp, _u64]),
     "pp_mlp_set_params": (ctypes.c_int, [_vp, _vp, _u64]),
+    "pp_mlp_set_activation": (ctypes.c_int, [_vp, _u32]),
     "pp_mlp_forward_device": (ctypes.c_int, [_vp, _vp, _u64, _vp]),
     "pp_mlp_forward": (ctypes.c_int, [_vp, _vp, _u64, _vp]),
     "pp_mlp_last_forward_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),

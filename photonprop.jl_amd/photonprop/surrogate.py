@@ -36,6 +36,8 @@ from .propagation import C_VACUUM, make_requests
 from .spectrum import InverseSquareSpectrum
 
 N_IN, WIDTH, DEPTH, N_OUT = 8, 256, 4, 10
+ACTIVATIONS = {"silu": _abi.PP_MLP_SILU, "relu": _abi.PP_MLP_RELU}
+ACTIVATION = "silu"  # hidden activation of the networks below unless one is named
 
 # Training domain and affine scaling of each feature: (lo, hi).
 DOMAIN = {
@@ -163,15 +165,25 @@ def generate_training_set(ctx, n_rows, photons_per_row=200_000, rows_per_medium=
 
 
 class MLP:
-    """The 8 -> 256^4 -> 10 surrogate network on the GPU (pp_mlp_*)."""
+    """The 8 -> 256^4 -> 10 surrogate network on the GPU (pp_mlp_*).
+    `activation` is the hidden activation: "relu" (the reference's network,
+    src/modelling.jl:212-220) or "silu"."""
 
-    def __init__(self, ctx, seed=0, n_in=N_IN, width=WIDTH, depth=DEPTH, n_out=N_OUT):
+    def __init__(self, ctx, seed=0, n_in=N_IN, width=WIDTH, depth=DEPTH, n_out=N_OUT,
+                 activation=None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.n_in, self.n_out = n_in, n_out
+        self.activation = activation or ACTIVATION
+        if self.activation not in ACTIVATIONS:
+            raise ValueError("unknown activation %r (one of %s)" % (activation, list(ACTIVATIONS)))
         self._h = ctypes.c_void_p()
         _abi.check(self.lib.pp_mlp_create(ctx.handle, n_in, width, depth, n_out, int(seed),
                                           ctypes.byref(self._h)), self.lib)
+        rc = self.lib.pp_mlp_set_activation(self._h, ACTIVATIONS[self.activation])
+        if rc != _abi.PP_OK:
+            self.close()
+            _abi.check(rc, self.lib)
         n = ctypes.c_uint64()
         _abi.check(self.lib.pp_mlp_n_params(self._h, ctypes.byref(n)), self.lib)
         self.n_params = n.value
@@ -324,7 +336,7 @@ class Ensemble:
 
 
 def train_ensemble(ctx, ts, k=4, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0,
-                   bootstrap=True, log=None):
+                   bootstrap=True, log=None, activation=None):
     """Train k networks (different initialisation and order; each on its own
     bootstrap resample of the rows when `bootstrap`) concurrently, one host
     thread and one stream per network.  Returns (Ensemble, per-member loss
@@ -334,7 +346,7 @@ def train_ensemble(ctx, ts, k=4, epochs=40, batch=4096, lr=3e-3, time_weight=0.5
     data = device_data(ts)
     n = ts.x.shape[0]
     rng = np.random.default_rng(seed + 17)
-    members = [MLP(ctx, seed=seed * 7919 + 101 * i) for i in range(k)]
+    members = [MLP(ctx, seed=seed * 7919 + 101 * i, activation=activation) for i in range(k)]
     rows = [rng.integers(0, n, n) if bootstrap else None for _ in range(k)]
     hists, errors = [None] * k, []
 
@@ -373,7 +385,8 @@ def train_ensemble(ctx, ts, k=4, epochs=40, batch=4096, lr=3e-3, time_weight=0.5
             log("retraining diverged members %s (final losses %s)" % (bad, final[bad]))
         for i in bad:
             members[i].close()
-            members[i] = MLP(ctx, seed=seed * 7919 + 101 * i + 100003 * attempt)
+            members[i] = MLP(ctx, seed=seed * 7919 + 101 * i + 100003 * attempt,
+                             activation=activation)
         train_all(bad)
     final = np.array([h[-1][0] for h in hists], dtype=np.float64)
     med = np.median(final[np.isfinite(final)]) if np.isfinite(final).any() else 0.0

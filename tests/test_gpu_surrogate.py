@@ -30,7 +30,10 @@ def unpack(params, dims):
     return out
 
 
-def ref_forward(layers, x, requires_grad=False):
+ACTS = ("silu", "relu")
+
+
+def ref_forward(layers, x, requires_grad=False, act="silu"):
     """FP32 reference with the kernels' BF16 rounding of weights and of each
     layer's input activations."""
     a = bf(torch.from_numpy(x))
@@ -41,7 +44,7 @@ def ref_forward(layers, x, requires_grad=False):
         params += [Wb, bb]
         z = a @ Wb.T + bb
         if l + 1 < len(layers):
-            s = z * torch.sigmoid(z)
+            s = z * torch.sigmoid(z) if act == "silu" else torch.relu(z)
             a = s + (bf(s) - s).detach()  # BF16 activations, straight-through gradient
         else:
             return z, params
@@ -81,24 +84,50 @@ def to_dev(*arrs):
     return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrs]
 
 
-def test_forward_matches_torch_fp32(ctx):
-    mlp = sg.MLP(ctx, seed=1)
+@pytest.mark.parametrize("act", ACTS)
+def test_forward_matches_torch_fp32(ctx, act):
+    mlp = sg.MLP(ctx, seed=1, activation=act)
     dims = [8, 256, 256, 256, 256, 10]
     layers = unpack(mlp.get_params(), dims)
     rng = np.random.default_rng(0)
     for M in (1, 37, 1000, 4099):
         x = rng.uniform(-1, 1, (M, 8)).astype(np.float32)
         y = mlp.forward(x)
-        ref, _ = ref_forward(layers, x)
+        ref, _ = ref_forward(layers, x, act=act)
         ref = ref.detach().numpy()
         err = np.abs(y - ref).max() / max(np.abs(ref).max(), 1e-6)
-        say("forward M=%d max rel err %.2e" % (M, err))
+        say("%s forward M=%d max rel err %.2e" % (act, M, err))
         assert err < 2e-2
     mlp.close()
 
 
-def test_loss_and_gradients_match_torch_autograd(ctx):
-    mlp = sg.MLP(ctx, seed=2)
+@pytest.mark.parametrize("shape", [(5, 48, 2, 3), (8, 256, 4, 16), (20, 112, 3, 1),
+                                   (8, 512, 2, 10)])
+def test_forward_other_shapes_match_torch_fp32(ctx, shape):
+    """Widths that leave half a 32-feature tile (48, 112), an odd input width,
+    1 and 16 outputs, and a width past the fused kernel's limit (512, layer by
+    layer on the GEMM kernel)."""
+    n_in, width, depth, n_out = shape
+    mlp = sg.MLP(ctx, seed=3, n_in=n_in, width=width, depth=depth, n_out=n_out, activation="relu")
+    dims = [n_in] + [width] * depth + [n_out]
+    p = mlp.get_params()
+    rng = np.random.default_rng(11)
+    p[:] = rng.normal(0, 0.1, p.size).astype(np.float32)  # non-zero biases too
+    mlp.set_params(p)
+    layers = unpack(mlp.get_params(), dims)
+    for M in (3, 300):
+        x = rng.uniform(-1, 1, (M, n_in)).astype(np.float32)
+        y = mlp.forward(x)
+        ref = ref_forward(layers, x, act="relu")[0].detach().numpy()
+        err = np.abs(y - ref).max() / max(np.abs(ref).max(), 1e-6)
+        say("shape %s M=%d max rel err %.2e" % (shape, M, err))
+        assert y.shape == (M, n_out) and err < 2e-2
+    mlp.close()
+
+
+@pytest.mark.parametrize("act", ACTS)
+def test_loss_and_gradients_match_torch_autograd(ctx, act):
+    mlp = sg.MLP(ctx, seed=2, activation=act)
     dims = [8, 256, 256, 256, 256, 10]
     # make the output layer non-trivial so every mixture term gets gradient
     p = mlp.get_params()
@@ -111,7 +140,7 @@ def test_loss_and_gradients_match_torch_autograd(ctx):
     dx, dln, dk, dt = to_dev(x, logn, k, times)
     tw = 0.7
     loss, g = mlp.loss_grad(dx.data_ptr(), dln.data_ptr(), dk.data_ptr(), dt.data_ptr(), T, M, tw)
-    y, params = ref_forward(layers, x, requires_grad=True)
+    y, params = ref_forward(layers, x, requires_grad=True, act=act)
     L, lp, lt = ref_loss(y, torch.from_numpy(logn).double(), torch.from_numpy(k).double(),
                          torch.from_numpy(times).double(), tw)
     L.backward()
@@ -127,7 +156,7 @@ def test_loss_and_gradients_match_torch_autograd(ctx):
             o += n
             cos = float(gl @ gr / (gl.norm() * gr.norm() + 1e-30))
             rel = float((gl.norm() - gr.norm()).abs() / (gr.norm() + 1e-30))
-            say("layer %d grad n=%d cos %.5f |rel norm| %.3e" % (l, n, cos, rel))
+            say("%s layer %d grad n=%d cos %.5f |rel norm| %.3e" % (act, l, n, cos, rel))
             assert cos > 0.99 and rel < 0.05
     assert o == mlp.n_params
     mlp.close()
@@ -160,7 +189,8 @@ def test_loss_beyond_the_log_mean_cap(ctx):
     mlp.close()
 
 
-def test_training_fits_a_known_yield_function(ctx):
+@pytest.mark.parametrize("act", ACTS)
+def test_training_fits_a_known_yield_function(ctx, act):
     """Poisson rows with log-yield f(x) = -9 + 2 x0 - x1²: the Poisson NLL must
     approach its floor and the fitted log yield f within 0.1."""
     rng = np.random.default_rng(5)
@@ -171,14 +201,14 @@ def test_training_fits_a_known_yield_function(ctx):
     k = rng.poisson(np.exp(f + logn)).astype(np.float32)
     times = -np.ones((M, 4), dtype=np.float32)
     ts = sg.TrainingSet(raw=x, x=x, log_n=logn, k=k, times=times)
-    mlp = sg.MLP(ctx, seed=4)
+    mlp = sg.MLP(ctx, seed=4, activation=act)
     hist = sg.train(mlp, ts, epochs=6, batch=2048, lr=3e-3, time_weight=0.0, log=say)
     assert hist[-1][0] < hist[0][0]
     xt = rng.uniform(-0.9, 0.9, (4096, 8)).astype(np.float32)
     ft = -9.0 + 2.0 * xt[:, 0] - xt[:, 1] ** 2
     y = mlp.forward(xt)
     err = np.median(np.abs(y[:, 0] - ft))
-    say("median |log-yield error| %.4f" % err)
+    say("%s median |log-yield error| %.4f" % (act, err))
     assert err < 0.1
     mlp.close()
 
