@@ -159,7 +159,7 @@ constexpr uint32_t kFusedRows = 128;      // 4 sample tiles of 32
 constexpr uint32_t kFusedMaxWidth = 256;  // 8 feature tiles: one per wave
 constexpr uint32_t kFusedLd = kFusedMaxWidth + 8;  // LDS row stride (bf16): rows 16 B apart
 constexpr uint32_t kFusedThreads = 512;
-constexpr int kFusedAhead = 4;            // weight fragments in flight per wave
+constexpr int kFusedAhead = 2;  // weight fragments in flight per wave (3 or 4 spill VGPRs)
 
 struct FusedArgs {
     const float* x;
@@ -200,13 +200,24 @@ __host__ __device__ inline FragGeom frag_geom(uint32_t no_pad, uint32_t k_pad, b
 // for it there).  The loads are unconditional (a lane past the layer's
 // features reads row 0: its accumulator rows are output features the epilogue
 // does not keep), so the code is straight-line.
+// li·kFusedLd + col as an opaque value.  Added to it, the constant offsets of
+// the other row tiles (t·32·kFusedLd) become immediate offsets of the LDS
+// instructions; left visible, the compiler folds li·kFusedLd + t·32·kFusedLd
+// back into (li | 32t)·kFusedLd, one product per row tile, and those stayed
+// live across the layer loop and spilled to scratch.
+__device__ __forceinline__ uint32_t lane_row_offset(uint32_t li, uint32_t col) {
+    uint32_t o = li * kFusedLd + col;
+    asm volatile("" : "+v"(o));
+    return o;
+}
+
 template <int kSteps>
 __device__ __forceinline__ void fused_mfma_steps(const bf16x8* wfr, const bf16* act, uint32_t li,
                                                  uint32_t kh, f32x16 (&acc)[4]) {
     bf16x8 wf[kFusedAhead];
 #pragma unroll
     for (int u = 0; u < kFusedAhead && u < kSteps; ++u) wf[u] = wfr[u * 64];
-    const bf16* arow = act + li * kFusedLd + kh;
+    const bf16* arow = act + lane_row_offset(li, kh);
     bf16x8 xb[2][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -240,9 +251,8 @@ static_assert(kFusedRows == 16 * (kFusedThreads / 64), "one 16-row output tile p
 // B column l%16 with k = 8·(l/16) + 0..7 of each 32-wide k-step, and D rows
 // 4·(l/16) + 0..3 of column l%16.  K is a multiple of 16; a k-group past it
 // is zero.
-__device__ __forceinline__ void fused_output_layer(const FusedArgs& f, uint32_t l,
-                                                   const bf16* act, uint32_t wave, uint32_t lane,
-                                                   uint32_t row0) {
+__device__ __forceinline__ void fused_output_layer(const FusedArgs& f, uint32_t l, bf16* act,
+                                                   uint32_t wave, uint32_t lane, uint32_t row0) {
     const uint32_t K = f.pdim[l];
     const uint32_t c = lane & 15u, kg = (lane >> 4) * 8u;
     const uint32_t rtile = wave * 16u;
@@ -255,13 +265,19 @@ __device__ __forceinline__ void fused_output_layer(const FusedArgs& f, uint32_t 
         const bf16x8 b = k + kg < K ? *reinterpret_cast<const bf16x8*>(arow + k) : zero;
         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfr[s * 64u], b, d, 0, 0, 0);
     }
-    const uint32_t i = row0 + rtile + c;
-    if (i < f.M) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t j = (lane >> 4) * 4u + q;
-            if (j < f.n_out) f.out[(size_t)i * f.ld_out + j] = d[q] + f.bias[l][j];
-        }
+    // The tile goes out through this wave's own LDS rows (only this wave read
+    // them, and its reads have completed): D is row-scattered across lanes, so
+    // stored directly each 4-byte store would write a few bytes of many lines.
+    float* stage = reinterpret_cast<float*>(act + rtile * kFusedLd);  // [16 rows][16]
+    *reinterpret_cast<f32x4*>(stage + c * 16u + (lane >> 4) * 4u) = d;
+    // other lanes of this wave read what a lane wrote
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint32_t n_out = f.n_out, n_rows = min(16u, f.M - min(f.M, row0 + rtile));
+    for (uint32_t e = lane; e < n_rows * n_out; e += 64u) {
+        const uint32_t r = e / n_out, j = e - r * n_out;
+        f.out[(size_t)(row0 + rtile + r) * f.ld_out + j] = stage[r * 16u + j] + f.bias[l][j];
     }
 }
 
@@ -299,7 +315,7 @@ __device__ __forceinline__ void fused_hidden_layer(const FusedArgs& f, uint32_t 
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const bf16x8 xb = *reinterpret_cast<const bf16x8*>(
-                        act + (t * 32u + li) * kFusedLd + kk + kh);
+                        act + lane_row_offset(li, kh) + t * 32u * kFusedLd + kk);
                     acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, acc[t], 0, 0, 0);
                 }
             }
@@ -319,9 +335,11 @@ __device__ __forceinline__ void fused_hidden_layer(const FusedArgs& f, uint32_t 
 #pragma unroll
             for (int r = 0; r < 4; ++r) bj[g][r] = jg + r < n_true ? bias[jg + r] : 0.0f;
         }
+        // row li, first feature of this lane's D rows; row tiles and feature
+        // groups are constant offsets from it
+        bf16* const erow = act + lane_row_offset(li, wave * 32u + 4u * (lane >> 5));
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const uint32_t i = t * 32u + li;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 // D rows 8g + 4*(lane/32) + 0..3 of this wave's feature tile
@@ -335,7 +353,7 @@ __device__ __forceinline__ void fused_hidden_layer(const FusedArgs& f, uint32_t 
                     const float sv = kRelu ? fmaxf(v, 0.0f) : silu(v);
                     y[r] = (bf16)(jg + r < n_true ? sv : 0.0f);
                 }
-                *reinterpret_cast<bf16x4*>(act + i * kFusedLd + jg) = y;
+                *reinterpret_cast<bf16x4*>(erow + t * 32u * kFusedLd + 8u * g) = y;
             }
         }
     }
