@@ -370,6 +370,18 @@ int pp_mlp_loss_grad_device(pp_mlp* mlp, const float* d_x, const float* d_log_n,
 int pp_mlp_train_step_device(pp_mlp* mlp, const float* d_x, const float* d_log_n,
                              const float* d_k, const float* d_times, uint32_t n_times,
                              uint32_t n_rows, float lr, float time_weight, double* loss_out);
+/* n_steps Adam steps over consecutive minibatches, the inner loop of the
+ * reference's train_mlp (src/modelling.jl:253-261: gradient, then update, per
+ * minibatch): step i uses rows [i·batch, (i+1)·batch) of the device arrays
+ * (laid out as for pp_mlp_loss_grad_device) and learning rate lr[i] (host
+ * array of n_steps).  The steps are enqueued back to back with no host
+ * synchronisation between them; the result equals n_steps calls of
+ * pp_mlp_train_step_device bit for bit.  loss_out[2] (may be NULL) receives
+ * the mean over the steps of each step's mean loss. */
+int pp_mlp_train_steps_device(pp_mlp* mlp, const float* d_x, const float* d_log_n,
+                              const float* d_k, const float* d_times, uint32_t n_times,
+                              uint32_t batch, uint32_t n_steps, const float* lr,
+                              float time_weight, double* loss_out);
 
 /* ---- device self-tests of the kernel's own helper functions ---------------- */
 /* The reference checks its device helpers through tiny test kernels

@@ -71,71 +71,74 @@ __device__ __forceinline__ float hidden_act(float z, uint32_t act) {
     return act == PP_MLP_RELU ? fmaxf(z, 0.0f) : silu(z);
 }
 
-// 128x128 block tile, 4 waves of 64x64 (2x2 MFMA 32x32x16 tiles each).
+// 64x64 block tile, 4 waves of one 32x32 MFMA tile each.  The products here
+// are small (a 4096-row minibatch against 256-wide layers: 256 blocks at this
+// tile), so the kernel is bound by memory latency rather than MFMA rate: each
+// wave keeps kGemmAhead k-steps of its A and B fragments in flight (a rolling
+// window, refilled as each step's MFMA is issued).  The 128x128 tile it
+// replaces loaded each k-step's fragments right before their MFMAs, on 64
+// blocks, and took 16 µs per product.
+constexpr int kGemmAhead = 8;
+
 __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t i0 = blockIdx.y * 128u + (wave >> 1) * 64u;
-    const uint32_t j0 = blockIdx.x * 128u + (wave & 1u) * 64u;
+    const uint32_t i0 = blockIdx.y * 64u + (wave >> 1) * 32u;
+    const uint32_t j0 = blockIdx.x * 64u + (wave & 1u) * 32u;
     const uint32_t kb = blockIdx.z * g.k_split;
     const uint32_t ke = min(g.K, kb + g.k_split);
+    const uint32_t nst = ke > kb ? (ke - kb) / 16u : 0u;  // K is a multiple of 16
     const uint32_t r = lane & 31u, kh = (lane >> 5) * 8u;
-    f32x16 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[a][b][q] = 0.0f;
+    const bool rok = i0 + r < g.M, cok = j0 + r < g.N;
+    const bf16* pa = g.A + (size_t)(rok ? i0 + r : 0u) * g.lda + kb + kh;
+    const bf16* pb = g.B + (size_t)(cok ? j0 + r : 0u) * g.ldb + kb + kh;
     const bf16x8 zero = {};
-    for (uint32_t k = kb; k < ke; k += 16) {
-        bf16x8 fa[2], fb[2];
+    f32x16 acc;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const uint32_t row = i0 + t * 32u + r;
-            fa[t] = row < g.M ? *reinterpret_cast<const bf16x8*>(g.A + (size_t)row * g.lda + k + kh)
-                              : zero;
-            const uint32_t col = j0 + t * 32u + r;
-            fb[t] = col < g.N ? *reinterpret_cast<const bf16x8*>(g.B + (size_t)col * g.ldb + k + kh)
-                              : zero;
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+    bf16x8 fa[kGemmAhead], fb[kGemmAhead];
+#pragma unroll
+    for (int u = 0; u < kGemmAhead; ++u) {
+        const bool in = (uint32_t)u < nst;
+        fa[u] = in && rok ? *reinterpret_cast<const bf16x8*>(pa + u * 16) : zero;
+        fb[u] = in && cok ? *reinterpret_cast<const bf16x8*>(pb + u * 16) : zero;
+    }
+    for (uint32_t s0 = 0; s0 < nst; s0 += kGemmAhead) {
+#pragma unroll
+        for (int u = 0; u < kGemmAhead; ++u) {
+            const uint32_t st = s0 + u;
+            if (st < nst) {  // uniform
+                const bf16x8 a = fa[u], b = fb[u];
+                const uint32_t sn = st + kGemmAhead;
+                const bool in = sn < nst;
+                fa[u] = in && rok ? *reinterpret_cast<const bf16x8*>(pa + sn * 16u) : zero;
+                fb[u] = in && cok ? *reinterpret_cast<const bf16x8*>(pb + sn * 16u) : zero;
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
         }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
     }
     // D layout of v_mfma_f32_32x32x16: lane holds column lane%32, rows
     // 8*(q/4) + 4*(lane/32) + q%4 for q = 0..15.
+    const uint32_t j = j0 + r;
+    if (j >= g.N) return;
+    const float bj = (g.epi >= EPI_BIAS_ACT) ? g.bias[j] : 0.0f;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const uint32_t j = j0 + b * 32u + r;
-            if (j >= g.N) continue;
-            const float bj = (g.epi >= EPI_BIAS_ACT) ? g.bias[j] : 0.0f;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const uint32_t i = i0 + a * 32u + 8u * (q >> 2) + 4u * (lane >> 5) + (q & 3);
-                if (i >= g.M) continue;
-                const float v = acc[a][b][q];
-                switch (g.epi) {
-                case EPI_STORE: g.C[(size_t)i * g.ldc + j] = v; break;
-                case EPI_PARTIAL:
-                    g.C[(size_t)blockIdx.z * g.split_stride + (size_t)i * g.ldc + j] = v;
-                    break;
-                case EPI_BIAS_ACT: {
-                    const float z = v + bj;
-                    g.C[(size_t)i * g.ldc + j] = z;
-                    g.Y[(size_t)i * g.ldy + j] = (bf16)hidden_act(z, g.act);
-                    break;
-                }
-                case EPI_ACT_ONLY:
-                    g.Y[(size_t)i * g.ldy + j] = (bf16)hidden_act(v + bj, g.act);
-                    break;
-                default: g.C[(size_t)i * g.ldc + j] = v + bj; break;
-                }
-            }
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t i = i0 + 8u * (q >> 2) + 4u * (lane >> 5) + (q & 3);
+        if (i >= g.M) continue;
+        const float v = acc[q];
+        switch (g.epi) {
+        case EPI_STORE: g.C[(size_t)i * g.ldc + j] = v; break;
+        case EPI_PARTIAL: g.C[(size_t)blockIdx.z * g.split_stride + (size_t)i * g.ldc + j] = v; break;
+        case EPI_BIAS_ACT: {
+            const float z = v + bj;
+            g.C[(size_t)i * g.ldc + j] = z;
+            g.Y[(size_t)i * g.ldy + j] = (bf16)hidden_act(z, g.act);
+            break;
         }
+        case EPI_ACT_ONLY: g.Y[(size_t)i * g.ldy + j] = (bf16)hidden_act(v + bj, g.act); break;
+        default: g.C[(size_t)i * g.ldc + j] = v + bj; break;
+        }
+    }
 }
 
 // Fused inference: every layer of the MLP for a tile of kFusedRows rows in
@@ -430,19 +433,39 @@ __global__ void act_backward_kernel(const float* dA, const float* Z, uint32_t M,
     }
 }
 
-// db[j] = Σ_i dZ[i][j]  (one block per 64 columns).
-__global__ void __launch_bounds__(256) bias_grad_kernel(const bf16* dZ, uint32_t M, uint32_t N,
-                                                        uint32_t ld, float* db) {
-    __shared__ float part[4][64];
-    const uint32_t j = blockIdx.x * 64u + (threadIdx.x & 63u);
-    const uint32_t w = threadIdx.x >> 6;
+// db[j] = Σ_i dZ[i][j] in two passes with a fixed summation order (the same
+// gradient on every run).  Pass 1: block (x, y) sums rows [y·chunk, (y+1)·chunk)
+// of columns [64x, 64x + 64) into part[y][j]; pass 2 adds the partials in y
+// order.  (One block per 64 columns walking every row took 250 µs of a 4096-row
+// step: four blocks on the whole device.)
+__global__ void __launch_bounds__(256) bias_grad_partial_kernel(const bf16* dZ, uint32_t M,
+                                                                uint32_t N, uint32_t ld,
+                                                                uint32_t chunk, float* part) {
+    __shared__ float red[4][64];
+    const uint32_t c = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t j = blockIdx.x * 64u + c;
+    const uint32_t i0 = blockIdx.y * chunk, i1 = min(M, i0 + chunk);
     float s = 0.0f;
     if (j < N)
-        for (uint32_t i = w; i < M; i += 4) s += (float)dZ[(size_t)i * ld + j];
-    part[w][threadIdx.x & 63u] = s;
+        for (uint32_t i = i0 + w; i < i1; i += 4) s += (float)dZ[(size_t)i * ld + j];
+    red[w][c] = s;
     __syncthreads();
-    if (w == 0 && j < N) db[j] += part[0][threadIdx.x] + part[1][threadIdx.x] +
-                                  part[2][threadIdx.x] + part[3][threadIdx.x];
+    if (w == 0 && j < N)
+        part[(size_t)blockIdx.y * N + j] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+}
+
+// One wave per column: lane y holds partial y (parts <= 64), then a butterfly
+// of fixed shape.
+static_assert(kMaxSplits <= 64, "one partial per lane");
+__global__ void __launch_bounds__(256) bias_grad_sum_kernel(const float* part, uint32_t parts,
+                                                            uint32_t N, float* db) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (j >= N) return;  // wave-uniform
+    float s = lane < parts ? part[(size_t)lane * N + j] : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) db[j] += s;
 }
 
 __device__ float digamma(float x) {
@@ -613,6 +636,10 @@ __global__ void zero_f32_kernel(float* p, uint64_t n) {
         p[t] = 0.0f;
 }
 
+__global__ void add_f64_kernel(const double* src, uint32_t n, double* acc) {
+    if (threadIdx.x < n) acc[threadIdx.x] += src[threadIdx.x];
+}
+
 __global__ void zero_f64_kernel(double* p, uint32_t n) {
     if (threadIdx.x < n) p[threadIdx.x] = 0.0;
 }
@@ -631,14 +658,16 @@ struct pp_mlp {
     uint64_t off_w[kMaxLayers], off_b[kMaxLayers], n_params = 0;
     DevBuf w, grad, adam_m, adam_v;     // fp32 master / grads / Adam state (packed)
     DevBuf gpad;                        // fp32 padded weight-gradient scratch
+    DevBuf bpart;                       // fp32 bias-gradient partials [kMaxSplits][width]
     DevBuf wt16[kMaxLayers], w16[kMaxLayers];
     DevBuf wfrag[kMaxLayers];           // bf16 weights in fragment order (fused inference)
     // activations (grow with batch)
     uint32_t cap_rows = 0;
     DevBuf x16, act[kMaxLayers + 1], actT[kMaxLayers + 1], z[kMaxLayers + 1], dA, dZ, dZT, dY,
-        yout, loss;
+        yout, loss, loss_acc;
     uint64_t step = 0;
     uint32_t activation = PP_MLP_SILU;  // hidden activation
+    bool frags_stale = true;            // wfrag lags the weights
 };
 
 namespace {
@@ -666,6 +695,7 @@ int mlp_reserve(pp_mlp* m, uint32_t rows) {
     if ((rc = ensure(m->dY, (size_t)R * m->pdim[L] * sizeof(bf16)))) return rc;
     if ((rc = ensure(m->yout, (size_t)R * m->pdim[L] * sizeof(float)))) return rc;
     if ((rc = ensure(m->loss, 2 * sizeof(double)))) return rc;
+    if ((rc = ensure(m->loss_acc, 2 * sizeof(double)))) return rc;
     m->cap_rows = R;
     return PP_OK;
 }
@@ -692,7 +722,7 @@ int gemm(pp_mlp* m, const bf16* A, uint32_t lda, const bf16* B, uint32_t ldb, ui
     g.split_stride = split_stride;
     g.epi = epi;
     g.act = m->activation;
-    dim3 grid((N + 127) / 128, (M + 127) / 128, (K + per - 1) / per);
+    dim3 grid((N + 63) / 64, (M + 63) / 64, (K + per - 1) / per);
     if (splits_used) *splits_used = grid.z;
     gemm_nt_kernel<<<grid, 256, 0, m->stream>>>(g);
     PP_HIP(hipGetLastError());
@@ -706,11 +736,23 @@ int sync_weights(pp_mlp* m) {
         sync_weights_kernel<<<grid1d((uint64_t)ldn * ldk), 256, 0, m->stream>>>(
             (const float*)m->w.p + m->off_w[l], no, ni, (bf16*)m->wt16[l].p, ldk,
             (bf16*)m->w16[l].p, ldn, ldk, ldn);
+    }
+    PP_HIP(hipGetLastError());
+    m->frags_stale = true;  // repacked by the next fused forward, not after every training step
+    return PP_OK;
+}
+
+// The fused kernel's fragment-order weights from wt16, if the weights changed.
+int pack_fragments(pp_mlp* m) {
+    if (!m->frags_stale) return PP_OK;
+    for (uint32_t l = 0; l < m->n_layers; ++l) {
+        const uint32_t ldk = m->pdim[l], ldn = m->pdim[l + 1];
         const FragGeom g = frag_geom(ldn, ldk, l + 1 == m->n_layers);
         pack_fragments_kernel<<<grid1d((uint64_t)g.tiles * g.steps * 64u), 256, 0, m->stream>>>(
             (const bf16*)m->wt16[l].p, ldn, ldk, g, (bf16x8*)m->wfrag[l].p);
     }
     PP_HIP(hipGetLastError());
+    m->frags_stale = false;
     return PP_OK;
 }
 
@@ -750,6 +792,8 @@ bool fused_fits(const pp_mlp* m) {
 // All layers in one kernel (mlp_forward_fused_kernel); fp32 x [M][n_in] in,
 // fp32 y [M][n_out] out, both device pointers.  Times the kernel alone.
 int forward_fused(pp_mlp* m, const float* d_x, uint32_t M, float* d_y) {
+    int rc;
+    if ((rc = pack_fragments(m))) return rc;
     FusedArgs fa;
     memset(&fa, 0, sizeof fa);
     const uint32_t L = m->n_layers;
@@ -837,8 +881,14 @@ int loss_grad(pp_mlp* m, uint32_t M, const float* d_logn, const float* d_k, cons
             return rc;
         unpack_grad_splits_kernel<<<grid1d((uint64_t)N * m->dims[l]), 256, 0, m->stream>>>(
             gp, used, stride, N, m->dims[l], K, (float*)m->grad.p + m->off_w[l]);
-        bias_grad_kernel<<<(N + 63) / 64, 256, 0, m->stream>>>(dz, M, N, ldz,
-                                                              (float*)m->grad.p + m->off_b[l]);
+        {
+            const uint32_t parts = std::max(1u, std::min(kMaxSplits, M / 64u));
+            const uint32_t chunk = (M + parts - 1) / parts;
+            bias_grad_partial_kernel<<<dim3((N + 63) / 64, parts), 256, 0, m->stream>>>(
+                dz, M, N, ldz, chunk, (float*)m->bpart.p);
+            bias_grad_sum_kernel<<<(N + 3) / 4, 256, 0, m->stream>>>(
+                (const float*)m->bpart.p, parts, N, (float*)m->grad.p + m->off_b[l]);
+        }
         if (l == 0) break;
         // dA_l = dZ · Wt_l  (B operand = W_l = Wt_l^T, [K][Np])
         if ((rc = gemm(m, dz, ldz, (const bf16*)m->w16[l].p, Np, M, K, Np, EPI_STORE,
@@ -851,6 +901,19 @@ int loss_grad(pp_mlp* m, uint32_t M, const float* d_logn, const float* d_k, cons
     }
     PP_HIP(hipGetLastError());
     return PP_OK;
+}
+
+// One Adam step (beta 0.9/0.999) from m->grad, then the bf16 weight copies;
+// enqueued on the network's stream.
+int adam_update(pp_mlp* m, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    ++m->step;
+    const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
+    adam_kernel<<<grid1d(m->n_params), 256, 0, m->stream>>>(
+        (float*)m->w.p, (const float*)m->grad.p, (float*)m->adam_m.p, (float*)m->adam_v.p,
+        m->n_params, lr, b1, b2, 1e-8f, c1, c2);
+    PP_HIP(hipGetLastError());
+    return sync_weights(m);
 }
 
 }  // namespace
@@ -905,6 +968,9 @@ int pp_mlp_create(pp_ctx* ctx, uint32_t n_in, uint32_t width, uint32_t depth, ui
             return bail(fail(PP_ERR_DEVICE, "hipMemset failed"));
     }
     if ((rc = ensure(m->gpad, gmax * 4 * kMaxSplits))) return bail(rc);
+    uint32_t nmax = 0;
+    for (uint32_t l = 1; l <= m->n_layers; ++l) nmax = std::max(nmax, m->dims[l]);
+    if ((rc = ensure(m->bpart, (size_t)nmax * 4 * kMaxSplits))) return bail(rc);
     // He-style uniform init, biases zero.
     std::vector<float> h(off, 0.0f);
     std::mt19937_64 rng(seed);
@@ -927,8 +993,8 @@ int pp_mlp_create(pp_ctx* ctx, uint32_t n_in, uint32_t width, uint32_t depth, ui
 int pp_mlp_destroy(pp_mlp* m) {
     if (!m) return PP_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    DevBuf* bufs[] = {&m->w, &m->grad, &m->adam_m, &m->adam_v, &m->gpad, &m->x16, &m->dA,
-                      &m->dZ, &m->dZT, &m->dY, &m->yout, &m->loss};
+    DevBuf* bufs[] = {&m->w, &m->grad, &m->adam_m, &m->adam_v, &m->gpad, &m->bpart, &m->x16, &m->dA,
+                      &m->dZ, &m->dZT, &m->dY, &m->yout, &m->loss, &m->loss_acc};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int l = 0; l <= kMaxLayers; ++l) {
@@ -1055,15 +1121,39 @@ int pp_mlp_train_step_device(pp_mlp* m, const float* d_x, const float* d_logn, c
     int rc = pp_mlp_loss_grad_device(m, d_x, d_logn, d_k, d_times, n_times, n_rows, time_weight,
                                      nullptr, loss_out);
     if (rc) return rc;
-    const float b1 = 0.9f, b2 = 0.999f;
-    ++m->step;
-    const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
-    adam_kernel<<<grid1d(m->n_params), 256, 0, m->stream>>>(
-        (float*)m->w.p, (const float*)m->grad.p, (float*)m->adam_m.p, (float*)m->adam_v.p,
-        m->n_params, lr, b1, b2, 1e-8f, c1, c2);
-    PP_HIP(hipGetLastError());
-    if ((rc = sync_weights(m))) return rc;
+    if ((rc = adam_update(m, lr))) return rc;
     PP_HIP(hipStreamSynchronize(m->stream));
+    return PP_OK;
+}
+
+int pp_mlp_train_steps_device(pp_mlp* m, const float* d_x, const float* d_logn, const float* d_k,
+                              const float* d_times, uint32_t n_times, uint32_t batch,
+                              uint32_t n_steps, const float* lr, float time_weight,
+                              double* loss_out) {
+    if (!m || !d_x || !d_logn || !d_k || (n_times && !d_times) || !lr || batch == 0 ||
+        n_steps == 0)
+        return fail(PP_ERR_INVALID_ARG, "NULL argument, empty batch or no steps");
+    PP_HIP(hipSetDevice(m->ctx->device));
+    int rc;
+    if ((rc = mlp_reserve(m, batch))) return rc;
+    zero_f64_kernel<<<1, 64, 0, m->stream>>>((double*)m->loss_acc.p, 2);
+    for (uint32_t i = 0; i < n_steps; ++i) {
+        const size_t o = (size_t)i * batch;
+        if ((rc = pack_input(m, d_x + o * m->dims[0], batch))) return rc;
+        if ((rc = loss_grad(m, batch, d_logn + o, d_k + o, n_times ? d_times + o * n_times : nullptr,
+                            n_times, time_weight)))
+            return rc;
+        add_f64_kernel<<<1, 64, 0, m->stream>>>((const double*)m->loss.p, 2,
+                                                 (double*)m->loss_acc.p);
+        if ((rc = adam_update(m, lr[i]))) return rc;
+    }
+    double l[2];
+    PP_HIP(hipMemcpyAsync(l, m->loss_acc.p, sizeof l, hipMemcpyDeviceToHost, m->stream));
+    PP_HIP(hipStreamSynchronize(m->stream));
+    if (loss_out) {
+        loss_out[0] = l[0] / ((double)batch * n_steps);
+        loss_out[1] = l[1] / ((double)batch * n_steps);
+    }
     return PP_OK;
 }
 

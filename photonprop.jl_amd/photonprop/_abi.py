@@ -199,6 +199,8 @@ SIGNATURES = {
                                                _vp, _vp]),
     "pp_mlp_train_step_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, ctypes.c_float,
                                                 ctypes.c_float, _vp]),
+    "pp_mlp_train_steps_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp,
+                                                 ctypes.c_float, _vp]),
     "pp_selftest_rotate": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64]),
     "pp_selftest_hg": (ctypes.c_int, [_vp, ctypes.c_float, _u64, _vp, _u64]),
     "pp_selftest_philox": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64]),

@@ -239,6 +239,17 @@ class MLP:
             _abi.ptr(loss)), self.lib)
         return loss
 
+    def train_steps(self, d_x, d_logn, d_k, d_times, n_times, batch, lrs, time_weight=1.0):
+        """len(lrs) Adam steps over consecutive minibatches of `batch` rows,
+        enqueued with no host synchronisation between them; mean loss."""
+        lrs = np.ascontiguousarray(lrs, dtype=np.float32)
+        loss = np.zeros(2)
+        _abi.check(self.lib.pp_mlp_train_steps_device(
+            self._h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_logn), ctypes.c_void_p(d_k),
+            ctypes.c_void_p(d_times), int(n_times), int(batch), int(lrs.size), _abi.ptr(lrs),
+            float(time_weight), _abi.ptr(loss)), self.lib)
+        return loss
+
 
 def device_data(ts, dev="cuda:0"):
     """The training set's arrays as device tensors (torch as plumbing)."""
@@ -275,7 +286,8 @@ def train(mlp, ts, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0, log=
     steps_total = epochs * (n // batch)
     step = 0
     hist = []
-    row_bytes = [t.stride(0) * t.element_size() for t in (X, LN, K, T)]
+    # train_steps takes each minibatch as the next block of rows
+    assert X.stride(0) == X.shape[1] and T.stride(0) == T.shape[1]
     for ep in range(epochs):
         perm = base[torch.randperm(n, generator=g)].to(dev)
         # one gather per epoch; minibatches are contiguous row blocks of it
@@ -283,17 +295,14 @@ def train(mlp, ts, epochs=40, batch=4096, lr=3e-3, time_weight=0.5, seed=0, log=
         ptrs = [t.data_ptr() for t in (Xp, LNp, Kp, Tp)]
         # the library's stream does not wait for torch's: finish the gather first
         torch.cuda.current_stream().synchronize()
-        acc = np.zeros(2)
-        nb = 0
-        for b in range(n // batch):
-            o = b * batch
-            cur = lr * 0.5 * (1 + math.cos(math.pi * step / max(1, steps_total)))
-            acc += mlp.train_step(*(q + o * rb for q, rb in zip(ptrs, row_bytes)),
-                                  T.shape[1], batch, cur, time_weight)
-            nb += 1
-            step += 1
+        nb = n // batch
+        steps = step + np.arange(nb)
+        lrs = lr * 0.5 * (1 + np.cos(math.pi * steps / max(1, steps_total)))
+        # the epoch's minibatches in one call: no host round trip per step
+        mean = mlp.train_steps(*ptrs, T.shape[1], batch, lrs, time_weight) if nb else np.zeros(2)
+        step += nb
         del Xp, LNp, Kp, Tp
-        hist.append((acc / max(nb, 1)).tolist())
+        hist.append(mean.tolist())
         if log:
             log("epoch %d: poisson %.5f time %.5f" % (ep, hist[-1][0], hist[-1][1]))
     return hist

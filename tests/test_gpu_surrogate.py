@@ -282,3 +282,31 @@ def test_training_is_reproducible(ctx):
         params.append(mlp.get_params())
         mlp.close()
     assert np.array_equal(params[0], params[1])
+
+
+def test_train_steps_equal_single_steps(ctx):
+    """pp_mlp_train_steps_device (a run of minibatch steps with no host
+    synchronisation between them) leaves the network bit for bit where the same
+    steps taken one call at a time leave it, with the same mean loss."""
+    rng = np.random.default_rng(13)
+    B, S, T = 1024, 5, 4
+    x, logn, k, times = batch(rng, B * S, T)
+    dx, dln, dk, dt = to_dev(x, logn, k, times)
+    lrs = np.float32(3e-3) * np.linspace(1.0, 0.5, S, dtype=np.float32)
+    one = sg.MLP(ctx, seed=21)
+    many = sg.MLP(ctx, seed=21)
+    losses = []
+    for i in range(S):
+        o = i * B
+        losses.append(one.train_step(dx.data_ptr() + 4 * o * 8, dln.data_ptr() + 4 * o,
+                                     dk.data_ptr() + 4 * o, dt.data_ptr() + 4 * o * T, T, B,
+                                     float(lrs[i]), 0.5))
+    mean = many.train_steps(dx.data_ptr(), dln.data_ptr(), dk.data_ptr(), dt.data_ptr(), T, B,
+                            lrs, 0.5)
+    assert np.array_equal(one.get_params(), many.get_params())
+    np.testing.assert_allclose(mean, np.mean(losses, axis=0), rtol=1e-12)
+    with pytest.raises(pp.PhotonPropError):
+        many.train_steps(dx.data_ptr(), dln.data_ptr(), dk.data_ptr(), dt.data_ptr(), T, B,
+                         np.zeros(0, np.float32), 0.5)
+    one.close()
+    many.close()
