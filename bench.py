@@ -237,18 +237,22 @@ def bench_serving(pp, ctx, spectrum, n_req):
     }
 
 
-def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log_std=0.05,
-                    n_eval=1024, eval_photons=100_000_000, activation=None):
+def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log_std=0.1,
+                    n_eval=1024, eval_target_hits=1000.0, eval_photons=(100_000_000, 2_000_000_000),
+                    activation=None):
     """Train the BF16-MFMA surrogate (an ensemble of `members` networks on bootstrap
     resamples of the MC rows from the batched kernel), then evaluate it on
     held-out pairs against high-statistics MC: yield errors over every pair with
     a truth hit, next to the truth's own error bar, and over the pairs resolved
-    to <= 5 %; the variance gate's fallback fraction, the error of the pairs it
-    keeps and the share of the worst-error decile it sends to MC; the hybrid's
-    answers; the median Kolmogorov distance between the predicted arrival-time
-    mixture and the MC residual times.  Speed: ensemble queries/s (forward of
-    1e6 rows per member, HIP events) against MC queries/s at the 4096-pair
-    batch's mean budget."""
+    to <= 5 %; the variance gate's fallback fraction and the error of the pairs
+    it keeps; the hybrid's answers and their cost; the median Kolmogorov
+    distance between the predicted arrival-time mixture and the MC residual
+    times.  Speed: ensemble queries/s (forward of 1e6 rows per member, HIP
+    events) against MC queries/s at the 4096-pair batch's mean budget.
+
+    Photons per training row (and per held-out pair) come from a pilot
+    network's yield estimate ŷ: clip(20 / ŷ, photons_per_row, 10·photons_per_row)
+    for training, clip(eval_target_hits / ŷ, *eval_photons) for the truth."""
     import torch
     from scipy.special import gammainc
 
@@ -257,12 +261,16 @@ def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log
 
     out = {}
     t0 = time.perf_counter()
-    ts = sg.generate_training_set(ctx, n_rows, photons_per_row=photons_per_row, n_times=16, seed=3)
+    pilot = sg.pilot_allocator(ctx, n_min=photons_per_row, n_max=10 * photons_per_row, seed=11,
+                               activation=activation)
+    ts = sg.generate_training_set(ctx, n_rows, n_times=16, seed=3, allocate=pilot)
     out["training_rows"] = n_rows
-    out["photons_per_row"] = photons_per_row
+    out["training_photons"] = ts.photons
+    out["photons_per_row"] = [photons_per_row, 10 * photons_per_row]
     out["datagen_s"] = time.perf_counter() - t0
-    print("surrogate: %d rows generated in %.1f s" % (n_rows, out["datagen_s"]), file=sys.stderr,
-          flush=True)
+    print("surrogate: %d rows (%.3g photons) generated in %.1f s" % (n_rows, ts.photons,
+                                                                     out["datagen_s"]),
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ens, hists = sg.train_ensemble(ctx, ts, k=members, epochs=epochs, batch=4096, lr=3e-3,
                                    time_weight=0.5, seed=0, activation=activation)
@@ -274,11 +282,25 @@ def bench_surrogate(pp, ctx, n_rows, photons_per_row, epochs, members=4, max_log
     print("surrogate: %d networks trained in %.1f s" % (members, out["train_s"]), file=sys.stderr,
           flush=True)
     del ts
-    pairs = sg.held_out_pairs(ctx, n_eval, eval_photons, seed=1234, per_request_cap=2000)
-    hy = sg.HybridYield(ens, ctx, max_log_std=max_log_std, mc_photons=10_000_000)
-    _, rep = sg.evaluate_hybrid(hy, pairs, eval_photons)
-    out["eval_photons_per_pair"] = eval_photons
+    truth_alloc = sg.YieldAllocator(pilot.model, target_hits=eval_target_hits,
+                                    n_min=eval_photons[0], n_max=eval_photons[1])
+    pairs = sg.held_out_pairs(ctx, n_eval, None, seed=1234, per_request_cap=2000,
+                              allocate=truth_alloc)
+    pilot.model.close()
+    hy = sg.HybridYield(ens, ctx, max_log_std=max_log_std)
+    _, rep = sg.evaluate_hybrid(hy, pairs)
+    out["eval_photons_per_pair"] = {
+        "rule": "clip(%g / pilot yield, %g, %g)" % (eval_target_hits, *eval_photons),
+        "total": int(sum(int(p["reqs"]["n_photons"].sum()) for p in pairs))}
     out["eval"] = rep
+    # the hybrid against plain MC at the same per-pair budget rule for every pair
+    rate = rep["hybrid_mc_photons"] / rep["hybrid_mc_kernel_s"] if rep["hybrid_mc_kernel_s"] else None
+    out["hybrid_queries_per_s"] = rep["pairs"] / rep["hybrid_s"]
+    out["mc_same_rule_queries_per_s"] = (rep["pairs"] / (rep["all_mc_photons"] / rate)
+                                         if rate else None)
+    out["hybrid_speedup_vs_mc_same_rule"] = (out["hybrid_queries_per_s"] /
+                                             out["mc_same_rule_queries_per_s"]
+                                             if rate else None)
     ks_d = []
     for p in pairs:
         _, _, ys = ens.predict(sg.scale_features(p["raw"]))

@@ -122,15 +122,18 @@ class TrainingSet:
 
 
 def generate_training_set(ctx, n_rows, photons_per_row=200_000, rows_per_medium=4096, n_times=16,
-                          seed=0, spectrum=None):
+                          seed=0, spectrum=None, allocate=None):
     """MC rows from the batched kernel: each group of rows_per_medium rows shares
     a randomly drawn medium; each row is one request of photons_per_row photons
     with up to n_times hit times kept (a subset independent of the photons'
-    histories)."""
+    histories).  `allocate` (optional) maps a group's raw features to each
+    row's photon count instead (see YieldAllocator); the Poisson likelihood
+    takes every row's own exposure, so rows may differ."""
     spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
     rng = np.random.default_rng(seed)
     raws, lns, ks, ts = [], [], [], []
     steps = 0
+    photons = 0
     done = 0
     batch = 0
     while done < n_rows:
@@ -139,11 +142,13 @@ def generate_training_set(ctx, n_rows, photons_per_row=200_000, rows_per_medium=
         medium = medium_for(*ms)
         ctx.set_medium(medium)
         raw, reqs = random_rows(rng, n, ms)
-        reqs["n_photons"] = photons_per_row
+        n_ph = (np.asarray(allocate(raw), dtype=np.uint64) if allocate is not None
+                else np.full(n, int(photons_per_row), dtype=np.uint64))
+        reqs["n_photons"] = n_ph
         reqs["seed"] = seed * 1_000_003 + batch
         reqs["stream"] = np.arange(n, dtype=np.uint32)
         hits, off, sums = ctx.propagate_batch(reqs, spectrum, per_request_cap=n_times,
-                                              hits_cap=int(min(photons_per_row * n, 1 << 27)),
+                                              hits_cap=int(min(int(n_ph.sum()), 1 << 27)),
                                               allow_overflow=True)
         tg = t_geo(np.linalg.norm(reqs["center"], axis=1), reqs["radius"], medium)
         times = -np.ones((n, n_times), dtype=np.float32)
@@ -152,16 +157,47 @@ def generate_training_set(ctx, n_rows, photons_per_row=200_000, rows_per_medium=
         slot = np.arange(hits.shape[0]) - np.repeat(off[:-1].astype(np.int64), cnt)
         times[row, slot] = np.maximum(hits["time"].astype(np.float64) - tg[row], 0.0)
         raws.append(raw)
-        lns.append(np.full(n, math.log(photons_per_row), dtype=np.float32))
+        lns.append(np.log(n_ph.astype(np.float64)).astype(np.float32))
         ks.append(sums["sum_w"].astype(np.float32))
         ts.append(times)
         steps += int(sums["n_steps"].sum())
+        photons += int(n_ph.sum())
         done += n
         batch += 1
     raw = np.concatenate(raws)
     return TrainingSet(raw=raw, x=scale_features(raw), log_n=np.concatenate(lns),
                        k=np.concatenate(ks), times=np.concatenate(ts), steps=steps,
-                       photons=int(photons_per_row) * n_rows)
+                       photons=photons)
+
+
+class YieldAllocator:
+    """Photons per training row from a pilot network's yield estimate ŷ:
+    clip(target_hits / ŷ, n_min, n_max), so that rows of low yield carry about
+    as many hits as rows of high yield (up to n_max).  The pilot is trained on
+    a separate MC set, so a row's own count never decides its exposure."""
+
+    def __init__(self, model, target_hits=20.0, n_min=100_000, n_max=1_000_000):
+        self.model = model
+        self.target_hits = float(target_hits)
+        self.n_min = int(n_min)
+        self.n_max = int(n_max)
+
+    def __call__(self, raw):
+        ly = self.model.forward(scale_features(raw))[:, 0].astype(np.float64)
+        n = self.target_hits * np.exp(-np.clip(ly, -60.0, 0.0))
+        return np.clip(np.rint(n), self.n_min, self.n_max).astype(np.uint64)
+
+
+def pilot_allocator(ctx, target_hits=20.0, n_min=100_000, n_max=1_000_000, rows=131_072,
+                    photons_per_row=100_000, epochs=12, seed=0, activation=None):
+    """A YieldAllocator whose pilot network is trained on `rows` MC rows of
+    photons_per_row photons each (drawn with their own seed).  The caller
+    closes allocator.model when done."""
+    ts = generate_training_set(ctx, rows, photons_per_row=photons_per_row, n_times=4,
+                               seed=seed + 7_777)
+    mlp = MLP(ctx, seed=seed + 99, activation=activation)
+    train(mlp, ts, epochs=epochs, batch=4096, lr=3e-3, time_weight=0.0, seed=seed + 5)
+    return YieldAllocator(mlp, target_hits=target_hits, n_min=n_min, n_max=n_max)
 
 
 class MLP:
@@ -423,17 +459,23 @@ def gamma_mixture(y):
 
 class HybridYield:
     """Yield per emitted photon from the surrogate, with the plain MC as the
-    fallback for rows out of the training domain, below the resolvable yield
-    floor, or (for an Ensemble) whose predicted spread of the log yield exceeds
-    max_log_std."""
+    fallback for rows out of the training domain, below the yield floor, or
+    (for an Ensemble) whose predicted spread of the log yield exceeds
+    max_log_std.  A fallback row runs mc_photons photons when that is given,
+    else clip(mc_target_hits / ŷ, mc_min, mc_max) with ŷ the surrogate's own
+    estimate, so that the MC answer carries about mc_target_hits hits (a
+    relative error near 1/sqrt(mc_target_hits)) wherever mc_max allows."""
 
-    def __init__(self, model, ctx, yield_floor=1e-7, max_log_std=0.05, mc_photons=10_000_000,
-                 spectrum=None):
+    def __init__(self, model, ctx, yield_floor=1e-8, max_log_std=0.1, mc_photons=None,
+                 mc_target_hits=400.0, mc_min=10_000_000, mc_max=1_000_000_000, spectrum=None):
         self.model = model
         self.ctx = ctx
         self.floor = yield_floor
         self.max_log_std = max_log_std
         self.mc_photons = mc_photons
+        self.mc_target_hits = float(mc_target_hits)
+        self.mc_min = int(mc_min)
+        self.mc_max = int(mc_max)
         self.spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
 
     def predict(self, raw):
@@ -448,6 +490,13 @@ class HybridYield:
         use_mc = ~in_domain(raw) | (yld < self.floor) | (std > self.max_log_std)
         return yld, std, use_mc
 
+    def mc_budget(self, yld):
+        """Photons per fallback row for surrogate estimates yld."""
+        if self.mc_photons is not None:
+            return np.full(np.shape(yld), int(self.mc_photons), dtype=np.uint64)
+        n = self.mc_target_hits / np.maximum(np.asarray(yld, dtype=np.float64), 1e-30)
+        return np.clip(np.rint(n), self.mc_min, self.mc_max).astype(np.uint64)
+
     def query(self, raw, reqs, medium, seed=None):
         """raw (n, 8) physical features and the matching requests (same medium).
         Returns (yield, used_mc mask, spread of the surrogate's log yield)."""
@@ -455,19 +504,21 @@ class HybridYield:
         if use_mc.any():
             self.ctx.set_medium(medium)
             r = reqs[use_mc].copy()
-            r["n_photons"] = self.mc_photons
+            r["n_photons"] = self.mc_budget(yld[use_mc])
             if seed is not None:
                 r["seed"] = seed
             _, _, sums = self.ctx.propagate_batch(r, self.spectrum, hits_cap=0, allow_overflow=True)
-            yld[use_mc] = sums["sum_w"] / self.mc_photons
+            yld[use_mc] = sums["sum_w"] / r["n_photons"].astype(np.float64)
         return yld, use_mc, std
 
 
-def held_out_pairs(ctx, n_pairs, photons, seed=1234, n_media=4, per_request_cap=0, spectrum=None):
+def held_out_pairs(ctx, n_pairs, photons, seed=1234, n_media=4, per_request_cap=0, spectrum=None,
+                   allocate=None):
     """High-statistics MC on held-out pairs: n_pairs random geometries spread
-    over n_media randomly drawn media, photons each.  Returns a list per medium
-    of dicts {medium, raw, reqs, hits, offsets, sums}; hits are kept up to
-    per_request_cap per pair."""
+    over n_media randomly drawn media, photons each (or allocate(raw) photons
+    per pair: see YieldAllocator).  Returns a list per medium of dicts {medium,
+    raw, reqs, hits, offsets, sums}; hits are kept up to per_request_cap per
+    pair; reqs["n_photons"] holds each pair's budget."""
     spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
     rng = np.random.default_rng(seed)
     out = []
@@ -477,7 +528,7 @@ def held_out_pairs(ctx, n_pairs, photons, seed=1234, n_media=4, per_request_cap=
         medium = medium_for(*ms)
         raw, reqs = random_rows(rng, per, ms)
         ctx.set_medium(medium)
-        reqs["n_photons"] = photons
+        reqs["n_photons"] = photons if allocate is None else allocate(raw)
         reqs["seed"] = seed * 1009 + g
         reqs["stream"] = np.arange(per, dtype=np.uint32)
         hits, off, sums = ctx.propagate_batch(
@@ -487,20 +538,33 @@ def held_out_pairs(ctx, n_pairs, photons, seed=1234, n_media=4, per_request_cap=
     return out
 
 
-def evaluate_hybrid(hy, pairs, truth_photons, mc_seed=4242):
+def evaluate_hybrid(hy, pairs, truth_photons=None, mc_seed=4242):
     """Surrogate and hybrid yields of held-out pairs against their MC truth.
 
-    Per pair: truth yield Σw/N, its relative error bar 1/sqrt(Σw) (an upper
+    Per pair: truth yield Σw/N (N = truth_photons, or the pair's own budget
+    when None), its relative error bar 1/sqrt(Σw) (an upper
     bound: every weight is <= 1, so Var Σw <= Σw), the surrogate yield and
     spread, the gate's decision and the hybrid's answer (the MC fallback runs
     with its own seed).  Returns (arrays, summary)."""
+    import time
+
     cols = {k: [] for k in ("truth", "truth_err", "sumw", "surrogate", "spread", "gated",
                             "hybrid", "in_domain", "below_floor")}
+    hybrid_s = mc_kernel_s = 0.0
+    mc_photons = all_mc_photons = 0
     for p in pairs:
         sur, std, gate = hy.predict(p["raw"])
+        t0 = time.perf_counter()
         hyb, used, _ = hy.query(p["raw"], p["reqs"], p["medium"], seed=mc_seed)
+        hybrid_s += time.perf_counter() - t0
+        if used.any():
+            mc_kernel_s += 1e-3 * hy.ctx.last_kernel_ms()
+            mc_photons += int(hy.mc_budget(sur[used]).sum())
+        all_mc_photons += int(hy.mc_budget(sur).sum())
         sw = p["sums"]["sum_w"].astype(np.float64)
-        cols["truth"].append(sw / truth_photons)
+        n_truth = (p["reqs"]["n_photons"].astype(np.float64) if truth_photons is None
+                   else float(truth_photons))
+        cols["truth"].append(sw / n_truth)
         cols["truth_err"].append(np.where(sw > 0, 1.0 / np.sqrt(np.maximum(sw, 1e-300)), np.inf))
         cols["sumw"].append(sw)
         cols["surrogate"].append(sur)
@@ -541,8 +605,19 @@ def evaluate_hybrid(hy, pairs, truth_photons, mc_seed=4242):
         "kept_median_rel_error": float(np.median(rel_s[keep])) if keep.any() else None,
         "kept_p90_rel_error": float(np.percentile(rel_s[keep], 90)) if keep.any() else None,
         "worst_decile_sent_to_mc": float(a["gated"][worst].mean()) if worst.any() else None,
+        # the gate picks out the worse answers: median surrogate error of the
+        # resolved pairs it sends to MC and of those it keeps
+        "gated_resolved_median_rel_error": (float(np.median(rel_s[res & a["gated"]]))
+                                            if (res & a["gated"]).any() else None),
         # the hybrid's answers
         "hybrid_median_rel_error_resolved": float(np.median(rel_h[res])),
         "hybrid_median_rel_error_all": float(np.median(rel_h[ok])),
+        # cost: wall time of the hybrid's queries (surrogate + MC fallback), the
+        # fallback's photons and kernel time, and the photons plain MC would
+        # run at the same budget rule for every pair
+        "hybrid_s": hybrid_s,
+        "hybrid_mc_photons": mc_photons,
+        "hybrid_mc_kernel_s": mc_kernel_s,
+        "all_mc_photons": all_mc_photons,
     }
     return a, s

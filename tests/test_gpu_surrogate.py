@@ -236,26 +236,34 @@ def test_mc_rows_train_and_hybrid_fallback():
 
 
 def test_ensemble_gate_against_held_out_mc():
-    """The surrogate's acceptance on held-out pairs, at a reduced training size:
-    four networks trained on bootstrap resamples of 2e5 MC rows, 1024 held-out
-    pairs in 4 media against 1e8-photon MC.  The gate (out of domain, below the
-    yield floor, or spread of the members' log yields above 0.05) must keep the
-    surrogate's own answers at a median yield error <= 5 %, send at least 90 %
-    of the worst-error decile to MC, and leave the hybrid's answers at a median
-    error <= 5 %."""
+    """The surrogate's acceptance on held-out pairs, at a reduced training size.
+    Four networks trained on bootstrap resamples of 5e5 MC rows whose photon
+    counts come from a pilot network's yield estimate ŷ (clip(20/ŷ, 1e5, 1e6)
+    photons per row); 1024 held-out pairs in 4 media against MC of
+    clip(1000/ŷ, 1e8, 2e9) photons each.  The gate (out of domain, yield below
+    1e-8, or spread of the members' log yields above 0.1) must keep the
+    surrogate's own answers at a median yield error <= 5 %, send to MC pairs
+    whose median error is larger than that of the pairs it keeps, fall back on
+    at most 60 % of the pairs (about a third at the full 2e6 rows), and leave
+    the hybrid's answers at a median error <= 5 %."""
     with pp.Context(pp.SeawaterConfigMedium()) as c:
-        ts = sg.generate_training_set(c, 200_000, photons_per_row=100_000, n_times=16, seed=3)
+        pilot = sg.pilot_allocator(c, seed=11)
+        ts = sg.generate_training_set(c, 500_000, n_times=16, seed=3, allocate=pilot)
         ens, hists = sg.train_ensemble(c, ts, k=4, epochs=40, seed=0)
         del ts
         assert all(h[-1][0] < h[0][0] for h in hists)
-        pairs = sg.held_out_pairs(c, 1024, 100_000_000, seed=1234)
-        hy = sg.HybridYield(ens, c, max_log_std=0.05, mc_photons=100_000_000)
-        a, s = sg.evaluate_hybrid(hy, pairs, 100_000_000)
+        truth = sg.YieldAllocator(pilot.model, target_hits=1000.0, n_min=100_000_000,
+                                  n_max=2_000_000_000)
+        pairs = sg.held_out_pairs(c, 1024, None, seed=1234, allocate=truth)
+        pilot.model.close()
+        hy = sg.HybridYield(ens, c)
+        a, s = sg.evaluate_hybrid(hy, pairs)
         say(s)
-        assert s["pairs_resolved"] >= 300
-        assert s["kept_resolved_pairs"] >= 10
+        assert s["pairs_resolved"] >= 500
+        assert s["kept_resolved_pairs"] >= 250
         assert s["kept_median_rel_error"] <= 0.05
-        assert s["worst_decile_sent_to_mc"] >= 0.9
+        assert s["gated_resolved_median_rel_error"] > s["kept_median_rel_error"]
+        assert s["fallback_fraction"] <= 0.6
         assert s["hybrid_median_rel_error_resolved"] <= 0.05
         # a wider gate keeps a superset of the pairs
         hy.max_log_std = 0.2
