@@ -144,46 +144,53 @@ __device__ __forceinline__ float hg_costheta_a(const HG& h, float a) {
 
 // rotate_to_axis (src/photon_prop_cuda.jl:171-219): apply to (nx,ny,nz) the
 // rotation taking e_z to (x,y,z), then renormalise.  The reference's form
-// divides by sinθ = sqrt(1-z²) twice; with q = (x·ny - y·nx)/(1+z) the same
-// rotation reads
-//     x' = nx·z + x·nz - y·q,  y' = ny·z + y·nz + x·q,  z' = nz·z - (x·nx + y·ny).
-// 1+z is taken as ((1+z)² + x² + y²)/2, equal to it for a unit vector
-// (x² + y² = (1-z)(1+z)) and free of cancellation at both poles: near z = -1
-// it follows x and y (as (x²+y²)/(1-z) would), near z = 1 it is 1+z.  For
-// |r|² = 1 + e the stand-in is off by e/2, like (x²+y²)/(1-z).  One
-// reciprocal and no select between the two forms.  (Its doubling stays a
-// multiply: the output modifier mul:2 is ignored in IEEE mode, which compute
-// kernels run in.)
+// divides by sinθ = sqrt(1-z²) twice.  Written as Rodrigues' rotation about
+// e_z × r = (-y, x, 0), with p = x·nx + y·ny and x² + y² = (1-z)(1+z), the
+// same rotation reads
+//     t = nz - p/(1+z),  x' = nx + x·t,  y' = ny + y·t,  z' = nz·z - p:
+// one reciprocal, one multiply, one fma for t and one fma per component.
+// 1/(1+z) is taken as 2/((1+z)² + x² + y²), equal to it for a unit vector and
+// free of cancellation at both poles: near z = -1 it follows x and y (as
+// (x²+y²)/(1-z) would), near z = 1 it is 1+z.  For |r|² = 1 + e the stand-in
+// is off by e/2, like (x²+y²)/(1-z).  Near z = -1, p/(1+z) grows like
+// 1/sqrt(1+z), but x·t and y·t stay bounded (x² + y² <= 2(1+z) + e), so the
+// result keeps absolute errors of a few ulp.  (The factor 2 rides in the fma
+// that forms t as the inline constant -2: the output modifier mul:2 is
+// ignored in IEEE mode, which compute kernels run in.)
 // y enters as y - 2^-60: that changes no y of magnitude above 2^-36 (the
 // subtraction rounds back to y), and at the exact south pole (x = y = 0,
 // z = -1), where the rotation is otherwise undetermined (0/0), it selects the
 // rotation by π about the x axis, so the result is the reference's
-// (nx, -ny, -nz) to rounding: x' = nx exactly, y' = -ny - 2^-60·nz,
-// z' = -nz + 2^-60·ny.  (x² + y² >= 2^-120 and |q| <= 2^61 stay in range.)
+// (nx, -ny, -nz) to rounding: x' = nx exactly, y' = -ny + 2^-60·(1 - nz),
+// z' = -nz + 2^-60·ny.  (x² + y² >= 2^-120 and |p/(1+z)| <= 2^61 stay in
+// range.)
 // Unit inputs give |r|² = 1 + e with |e| of a few 1e-6 (rounding and the
 // hardware sin/cos), so the renormalising 1/sqrt(|r|²) is one Newton step from
 // 1, 1.5 - |r|²/2, whose error 3e²/8 is far below an FP32 ulp; it needs no
 // transcendental.  kRenorm = false skips it: one rotation of a unit vector
 // moves |r| by ~2e-7, so a caller may renormalise every few rotations (after
-// four, |e| stays below ~1e-6 and the Newton step is still exact to ~1e-12).
-// kInBiased: the caller's y already carries the -2^-60; kOutBiased: the
-// result's y carries it too, folded into its first fma (an fmaak with the
-// literal), so a loop that feeds each result into the next rotation pays for
-// the bias with no instruction.
-template <bool kRenorm = true, bool kInBiased = false, bool kOutBiased = false>
-__device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, float nx, float ny,
-                                               float nz, float& ox, float& oy, float& oz) {
+// eight, |e| stays within a few 1e-6 and the Newton step is still exact).
+// kInBiased: the caller's y already carries the -2^-60.  kOutBiased: the
+// result's y carries a bias too, ny - 2^-60 taking the place of ny (and in p,
+// where it changes p by at most 2^-60), so y' = y·t + ny - 2^-60·(1 + t) when
+// y entered biased; rotate_polar folds it into the product that forms ny, so a
+// loop that feeds each result into the next rotation pays for the bias with
+// no instruction.
+template <bool kRenorm, bool kInBiased>
+__device__ __forceinline__ void rotate_core(float x, float y_in, float z, float nx, float ny,
+                                            float nz, float& ox, float& oy, float& oz) {
     const float y = kInBiased ? y_in : y_in - 0x1p-60f;
     const float s2 = __builtin_fmaf(y, y, x * x);
     const float zp = 1.0f + z;
-    const float inv_h = __builtin_amdgcn_rcpf(__builtin_fmaf(zp, zp, s2)) * 2.0f;
-    const float cr = __builtin_fmaf(x, ny, -(y * nx));
-    const float q = cr * inv_h;
-    const float dxy = __builtin_fmaf(x, nx, y * ny);
-    const float rx = __builtin_fmaf(nx, z, __builtin_fmaf(x, nz, -(y * q)));
-    const float ry = __builtin_fmaf(
-        ny, z, __builtin_fmaf(x, q, kOutBiased ? __builtin_fmaf(y, nz, -0x1p-60f) : y * nz));
-    const float rz = __builtin_fmaf(nz, z, -dxy);
+    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(zp, zp, s2));  // 1/(2(1+z))
+    const float p = __builtin_fmaf(x, nx, y * ny);
+    // -t = 2p/(1+z) - nz; the components take it negated (a source
+    // modifier), which keeps their fmas in the three-address form, so the
+    // results can land in the registers of x and y
+    const float tn = __builtin_fmaf(p * r, 2.0f, -nz);
+    const float rx = __builtin_fmaf(x, -tn, nx);
+    const float ry = __builtin_fmaf(y, -tn, ny);
+    const float rz = __builtin_fmaf(nz, z, -p);
     if (!kRenorm) {
         ox = rx;
         oy = ry;
@@ -195,6 +202,23 @@ __device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, flo
     ox = rx * inv;
     oy = ry * inv;
     oz = rz * inv;
+}
+
+template <bool kRenorm = true, bool kInBiased = false, bool kOutBiased = false>
+__device__ __forceinline__ void rotate_to_axis(float x, float y_in, float z, float nx, float ny,
+                                               float nz, float& ox, float& oy, float& oz) {
+    rotate_core<kRenorm, kInBiased>(x, y_in, z, nx, kOutBiased ? ny - 0x1p-60f : ny, nz, ox, oy,
+                                    oz);
+}
+
+// The same for n = (st·cφ, st·sφ, ct), the form every caller has: the output
+// bias is folded into the product that forms ny (one fmaak).
+template <bool kRenorm = true, bool kInBiased = false, bool kOutBiased = false>
+__device__ __forceinline__ void rotate_polar(float x, float y_in, float z, float cphi, float sphi,
+                                             float st, float ct, float& ox, float& oy, float& oz) {
+    const float nx = cphi * st;
+    const float ny = kOutBiased ? __builtin_fmaf(sphi, st, -0x1p-60f) : sphi * st;
+    rotate_core<kRenorm, kInBiased>(x, y_in, z, nx, ny, ct, ox, oy, oz);
 }
 
 // Renormalise a direction of squared length 1 + e, |e| of a few 1e-6: one

@@ -304,8 +304,8 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
                 const float st = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ct, ct, 1.0f));
                 // not renormalised here: step 0's rotation renormalises; y
                 // carries the step loop's bias
-                ppd::rotate_to_axis<false, false, true>(R.axis[0], R.axis[1], R.axis[2], cp * st,
-                                                        sp * st, ct, dx, dy, dz);
+                ppd::rotate_polar<false, false, true>(R.axis[0], R.axis[1], R.axis[2], cp, sp, st,
+                                                      ct, dx, dy, dz);
             }
         }
         // (dy carries a bias of -2^-60, which the rotations need at the exact
@@ -357,7 +357,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
             const float st = ppd::sin_from_cos(ct);                                     \
             const float sph = __builtin_amdgcn_sinf(PH);                                \
             const float cph = __builtin_amdgcn_cosf(PH);                                \
-            ppd::rotate_to_axis<false, true, true>(dx, dy, dz, cph * st, sph * st, ct, dx, dy, dz); \
+            ppd::rotate_polar<false, true, true>(dx, dy, dz, cph, sph, st, ct, dx, dy, dz);       \
             if (RN) ppd::renormalize(dx, dy, dz);                                       \
         }                                                                               \
     }
