@@ -128,43 +128,6 @@ def bench_batched(pp, ctx, spectrum, reps=3):
     }
 
 
-def per_request_open_loop(pp, ctx, spectrum, arrivals, reqs):
-    """One pp_propagate call per request (kernels, reduction, summary copy and
-    stream sync each time), served FIFO as the requests arrive (Poisson
-    schedule): latency = completion - arrival.  The C structs are built before
-    the schedule starts, so each request costs exactly one C-ABI call."""
-    import ctypes
-
-    from photonprop import propagation as prp
-
-    spec, keep = prp.spectrum_struct(spectrum)
-    ems, tgs = [], []
-    for r in reqs:
-        ems.append(prp.emitter_struct(pp.PhotonSource(tuple(r["pos"]), tuple(r["axis"]), 0.0,
-                                                      int(r["n_photons"]), spectrum)))
-        tgs.append(prp.sphere_struct(pp.DetectionSphere(tuple(r["center"]), float(r["radius"]))))
-    seeds = [int(x) for x in reqs["seed"]]
-    streams = [int(x) for x in reqs["stream"]]
-    summ = pp._abi.pp_summary()
-    lib, h = ctx.lib, ctx.handle
-    lat = np.zeros(len(reqs))
-    steps = 0
-    t0 = time.perf_counter()
-    for i in range(len(reqs)):
-        now = time.perf_counter() - t0
-        if now < arrivals[i]:
-            time.sleep(arrivals[i] - now)
-        rc = lib.pp_propagate(h, ctypes.byref(ems[i]), ctypes.byref(spec), ctypes.byref(tgs[i]),
-                              seeds[i], streams[i], 10, None, 0, ctypes.byref(summ))
-        if rc not in (0, pp._abi.PP_ERR_OVERFLOW):
-            pp._abi.check(rc, lib)
-        steps += summ.n_steps
-        lat[i] = 1e6 * (time.perf_counter() - t0 - arrivals[i])
-    del keep
-    return {"latency_us": lat, "makespan_s": time.perf_counter() - t0,
-            "photons": int(reqs["n_photons"].sum()), "steps": steps}
-
-
 def summarize(res, n_req):
     lat = res["latency_us"]
     return {
@@ -186,9 +149,9 @@ def _serving_sweep(pp, ctx, spectrum, n_req, rates, photons, label):
     """Open-loop Poisson sweep of one request mix through (a) the server,
     driven by the native load generator (libpploadgen.so: submitting and
     collecting threads in C++), and (b) one pp_propagate launch per request
-    (FIFO, one C-ABI call per request from Python).  Latency is measured from
-    the scheduled arrival in both.  A rate is sustained when p99 < 20 ms and at
-    least 95 % of the offered rate is delivered."""
+    (FIFO, driven by the same library's native per-request loop).  Latency is
+    measured from the scheduled arrival in both.  A rate is sustained when p99
+    < 20 ms and at least 95 % of the offered rate is delivered."""
     from photonprop import workloads
 
     out = {"photons_per_request": list(photons), "requests_per_rate": n_req, "server": {},
@@ -196,7 +159,7 @@ def _serving_sweep(pp, ctx, spectrum, n_req, rates, photons, label):
     for rate in rates:
         arr, reqs = workloads.serving_requests(n_req, rate, seed=11, photons=photons)
         out["per_request_launch"][str(int(rate))] = summarize(
-            per_request_open_loop(pp, ctx, spectrum, arr, reqs), n_req)
+            pp.run_per_request_native(ctx, spectrum, arr, reqs), n_req)
         print("%s per-request launch %d/s: %s" % (label, rate,
                                                    out["per_request_launch"][str(int(rate))]),
               file=sys.stderr, flush=True)

@@ -5,7 +5,10 @@
 // the results in submission order.  It exists so that measurements of the
 // server are not bounded by the caller's language runtime (a Python driver
 // tops out near 60,000 small requests/s: one interpreter lock for the
-// submitting and the collecting thread, and a sleep per request).
+// submitting and the collecting thread, and a sleep per request).  The same
+// schedule can be served by one pp_propagate call per request instead
+// (ppl_per_request_loop), so that both serving paths are measured from native
+// code.
 //
 // The serving entry points come in as function pointers, taken from the
 // library instance that created the server, so this file links against
@@ -27,6 +30,8 @@ typedef int (*submit_fn)(pp_server*, const pp_request*, uint64_t, uint64_t*);
 typedef int (*wait_fn)(pp_server*, uint64_t, uint32_t, int*);
 typedef int (*result_fn)(pp_server*, uint64_t, pp_summary*, pp_hit*, uint64_t, uint64_t*,
                          double*);
+typedef int (*propagate_fn)(pp_ctx*, const pp_emitter*, const pp_spectrum*, const pp_sphere*,
+                            uint64_t, uint32_t, uint32_t, pp_hit*, uint64_t, pp_summary*);
 
 using clk = std::chrono::steady_clock;
 
@@ -122,6 +127,50 @@ int ppl_open_loop(pp_server* srv, submit_fn submit, wait_fn wait, result_fn resu
     }
     abort.store(true);
     producer.join();
+    *makespan_s = since(t0);
+    return rc;
+}
+
+// The same schedule served by one propagate call (pp_propagate: kernels,
+// reduction, summary copy and stream synchronisation) per request, first in
+// first out on the calling thread: request i starts at max(arrivals_s[i], the
+// end of request i - 1).  latency_us[i] = its completion - arrivals_s[i];
+// steps[i] = photon steps; *makespan_s = start to the last completion.
+// Returns 0, the first negative library status other than PP_ERR_OVERFLOW
+// (no hits are kept, so every request with a hit overflows), or
+// PPL_ERR_TIMEOUT once timeout_s has passed.
+int ppl_per_request_loop(pp_ctx* ctx, propagate_fn propagate, const pp_spectrum* spectrum,
+                         const pp_request* reqs, const double* arrivals_s, uint64_t n,
+                         double timeout_s, double* latency_us, uint64_t* steps,
+                         double* makespan_s) {
+    if (!ctx || !propagate || !spectrum || (n && (!reqs || !arrivals_s || !latency_us || !steps)) ||
+        !makespan_s)
+        return PPL_ERR_ARG;
+    int rc = 0;
+    const clk::time_point t0 = clk::now();
+    for (uint64_t i = 0; i < n; ++i) {
+        for (;;) {
+            const double early = arrivals_s[i] - since(t0);
+            if (early <= 0.0) break;
+            if (early > 200e-6)
+                std::this_thread::sleep_for(std::chrono::duration<double>(early - 150e-6));
+        }
+        pp_summary s;
+        const pp_request& r = reqs[i];
+        const int p = propagate(ctx, &r.emitter, spectrum, &r.target, r.seed, r.stream,
+                                r.max_steps, nullptr, 0, &s);
+        if (p < 0 && p != PP_ERR_OVERFLOW) {
+            rc = p;
+            break;
+        }
+        const double now = since(t0);
+        latency_us[i] = 1e6 * (now - arrivals_s[i]);
+        steps[i] = s.n_steps;
+        if (now > timeout_s) {
+            rc = PPL_ERR_TIMEOUT;
+            break;
+        }
+    }
     *makespan_s = since(t0);
     return rc;
 }

@@ -14,6 +14,6 @@ from .medium import (MediumTables, SeawaterConfigMedium, WaterProperties,  # noq
                      make_cascadia_medium_properties, medium_tables)
 from .propagation import (Context, REQUEST_DTYPE, make_requests, propagate_distance,  # noqa: F401
                           request_from, result_table)
-from .serving import Server, run_open_loop, run_open_loop_native  # noqa: F401
+from .serving import Server, run_open_loop, run_open_loop_native, run_per_request_native  # noqa: F401
 from .spectrum import (CherenkovSpectrum, InverseSquareSpectrum, Monochromatic,  # noqa: F401
                        frank_tamm, frank_tamm_inverted_cdf, frank_tamm_norm)

@@ -163,6 +163,9 @@ def _loadgen():
         lg.ppl_open_loop.restype = ctypes.c_int
         lg.ppl_open_loop.argtypes = [vp, vp, vp, vp, vp, dp, ctypes.c_uint64, ctypes.c_uint64,
                                      ctypes.c_double, dp, dp, ctypes.POINTER(ctypes.c_uint64), dp]
+        lg.ppl_per_request_loop.restype = ctypes.c_int
+        lg.ppl_per_request_loop.argtypes = [vp, vp, vp, vp, dp, ctypes.c_uint64, ctypes.c_double,
+                                            dp, ctypes.POINTER(ctypes.c_uint64), dp]
         _LOADGEN = lg
     return _LOADGEN
 
@@ -194,6 +197,44 @@ def run_open_loop_native(server, arrivals, requests, hits_cap=0, timeout_s=120.0
     return {
         "latency_us": lat,
         "submit_lag_us": lag,
+        "makespan_s": mk.value,
+        "photons": int(np.asarray(reqs["n_photons"], dtype=np.uint64).sum()),
+        "steps": int(steps.sum()),
+        "steps_per_request": steps,
+    }
+
+
+def run_per_request_native(ctx, spectrum, arrivals, requests, timeout_s=300.0):
+    """The same open-loop schedule served by one pp_propagate call per request
+    (first in, first out) from native code (libpploadgen.so's
+    ppl_per_request_loop, through the entry point of the library instance that
+    created `ctx`).  Returns {latency_us, makespan_s, photons, steps,
+    steps_per_request}."""
+    lg = _loadgen()
+    lib = ctx.lib
+    reqs = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    n = reqs.shape[0]
+    lat = np.zeros(n)
+    steps = np.zeros(n, dtype=np.uint64)
+    mk = ctypes.c_double()
+    spec, keep = spectrum_struct(spectrum)
+    rc = lg.ppl_per_request_loop(ctx.handle, ctypes.cast(lib.pp_propagate, ctypes.c_void_p),
+                                 ctypes.cast(ctypes.byref(spec), ctypes.c_void_p),
+                                 reqs.ctypes.data_as(ctypes.c_void_p),
+                                 arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                 float(timeout_s),
+                                 lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 steps.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                 ctypes.byref(mk))
+    del keep
+    if rc == LOADGEN_TIMEOUT:
+        raise TimeoutError("per-request loop: not finished within %.0f s" % timeout_s)
+    if rc == LOADGEN_BAD_ARG:
+        raise ValueError("per-request loop: invalid arguments")
+    _abi.check(rc, lib)
+    return {
+        "latency_us": lat,
         "makespan_s": mk.value,
         "photons": int(np.asarray(reqs["n_photons"], dtype=np.uint64).sum()),
         "steps": int(steps.sum()),

@@ -161,6 +161,13 @@ def test_server_open_loop_native_driver(ctx):
         % (np.percentile(a["latency_us"], 50), np.percentile(a["latency_us"], 99), a["makespan_s"]))
     assert np.all(a["latency_us"] > 0)
     assert a["steps"] == b["steps"] > 9 * a["photons"]
+    # the same schedule as one pp_propagate call per request, from native code:
+    # every request's photon steps equal the server's
+    c = pp.run_per_request_native(ctx, spec, arr, reqs)
+    say("native per-request loop 5k req/s: p50 %.0f us p99 %.0f us, makespan %.3f s"
+        % (np.percentile(c["latency_us"], 50), np.percentile(c["latency_us"], 99), c["makespan_s"]))
+    assert np.all(c["latency_us"] > 0)
+    assert np.array_equal(c["steps_per_request"], a["steps_per_request"])
 
 
 def test_server_entry_ring_wraps(ctx):

@@ -48,10 +48,13 @@ def test_loadgen_library_loads_and_rejects_bad_arguments():
     from photonprop import serving
 
     lg = serving._loadgen()
-    assert hasattr(lg, "ppl_open_loop")
+    assert hasattr(lg, "ppl_open_loop") and hasattr(lg, "ppl_per_request_loop")
     mk = ctypes.c_double()
     rc = lg.ppl_open_loop(None, None, None, None, None, None, 0, 0, 1.0, None, None, None,
                           ctypes.byref(mk))
+    assert rc == serving.LOADGEN_BAD_ARG
+    rc = lg.ppl_per_request_loop(None, None, None, None, None, 0, 1.0, None, None,
+                                 ctypes.byref(mk))
     assert rc == serving.LOADGEN_BAD_ARG
 
 
