@@ -542,10 +542,16 @@ def evaluate_hybrid(hy, pairs, truth_photons=None, mc_seed=4242):
     """Surrogate and hybrid yields of held-out pairs against their MC truth.
 
     Per pair: truth yield Σw/N (N = truth_photons, or the pair's own budget
-    when None), its relative error bar 1/sqrt(Σw) (an upper
-    bound: every weight is <= 1, so Var Σw <= Σw), the surrogate yield and
-    spread, the gate's decision and the hybrid's answer (the MC fallback runs
-    with its own seed).  Returns (arrays, summary)."""
+    when None) and its relative error bar sqrt(Σw²)/Σw (the hit count is
+    Poisson, so Var Σw = E Σw²), with Σw² taken from the pair's stored hit
+    weights scaled by n_hits/n_stored (the stored records are a subset
+    independent of the photons' histories); a pair without stored hits gets
+    1/sqrt(Σw), an upper bound (every weight is <= 1).  A far pair's photons
+    arrive strongly absorbed, so its weights are small and 1/sqrt(Σw) can
+    overstate the error many times.  A pair is resolved when its error bar is
+    <= 5 %.  Also the surrogate yield and spread, the gate's decision and the
+    hybrid's answer (the MC fallback runs with its own seed).  Returns
+    (arrays, summary)."""
     import time
 
     cols = {k: [] for k in ("truth", "truth_err", "sumw", "surrogate", "spread", "gated",
@@ -565,7 +571,17 @@ def evaluate_hybrid(hy, pairs, truth_photons=None, mc_seed=4242):
         n_truth = (p["reqs"]["n_photons"].astype(np.float64) if truth_photons is None
                    else float(truth_photons))
         cols["truth"].append(sw / n_truth)
-        cols["truth_err"].append(np.where(sw > 0, 1.0 / np.sqrt(np.maximum(sw, 1e-300)), np.inf))
+        err = np.where(sw > 0, 1.0 / np.sqrt(np.maximum(sw, 1e-300)), np.inf)
+        stored = p["sums"]["n_stored"].astype(np.float64)
+        if p.get("hits") is not None and len(p["hits"]) and stored.sum() > 0:
+            cnt = np.diff(p["offsets"].astype(np.int64))
+            row = np.repeat(np.arange(len(sw)), cnt)
+            w = p["hits"]["weight"].astype(np.float64)
+            sw2 = np.bincount(row, weights=w * w, minlength=len(sw))
+            scale = p["sums"]["n_hits"].astype(np.float64) / np.maximum(stored, 1.0)
+            ok_h = (stored > 0) & (sw > 0)
+            err = np.where(ok_h, np.sqrt(sw2 * scale) / np.maximum(sw, 1e-300), err)
+        cols["truth_err"].append(err)
         cols["sumw"].append(sw)
         cols["surrogate"].append(sur)
         cols["spread"].append(std)
@@ -575,7 +591,7 @@ def evaluate_hybrid(hy, pairs, truth_photons=None, mc_seed=4242):
         cols["below_floor"].append(sur < hy.floor)
     a = {k: np.concatenate(v) for k, v in cols.items()}
     ok = a["sumw"] > 0
-    res = a["sumw"] >= 400.0  # truth known to <= 5 %
+    res = ok & (a["truth_err"] <= 0.05)  # truth known to <= 5 %
     rel_s = np.full(ok.shape, np.inf)
     rel_h = np.full(ok.shape, np.inf)
     rel_s[ok] = np.abs(a["surrogate"][ok] - a["truth"][ok]) / a["truth"][ok]
