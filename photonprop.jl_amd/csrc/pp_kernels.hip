@@ -266,19 +266,25 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
         const bool cone = R.profile == PP_EMIT_CHERENKOV_CONE;
         const float u_l = 2.0f - ppd::draw_a(w0.x);
         float lambda;
-        // the launch constants of the emission, loaded for this round only (no
-        // registers held for them through the step loop)
-        const TileCtx T = load_constant(opaque_uniform_ptr(Tc), 0);
-        if (T.spec.kind == PP_SPEC_MONO) {
-            lambda = T.spec.mono;
-        } else if (T.spec.kind == PP_SPEC_KNOTS) {
-            lambda = ppd::knot_interp(T.knots, T.spec.n_knots, T.spec.knot_step, u_l);
+        // the launch constants of the emission, loaded field by field for this
+        // round only (scalar loads; no registers held for them through the step
+        // loop; a whole-struct copy here became vector loads)
+        const TileCtx* const Tu = opaque_uniform_ptr(Tc);
+        const uint32_t spec_kind = load_constant(&Tu->spec.kind, 0);
+        if (spec_kind == PP_SPEC_MONO) {
+            lambda = load_constant(&Tu->spec.mono, 0);
+        } else if (spec_kind == PP_SPEC_KNOTS) {
+            lambda = ppd::knot_interp(load_constant(&Tu->knots, 0), load_constant(&Tu->spec.n_knots, 0),
+                                      load_constant(&Tu->spec.knot_step, 0), u_l);
         } else {
             // hardware reciprocal (1 ulp) rather than the correctly rounded
             // division sequence, here and for the cone angle below
-            lambda = __builtin_amdgcn_rcpf(__builtin_fmaf(-u_l, T.spec.d, T.spec.a));
+            lambda = __builtin_amdgcn_rcpf(
+                __builtin_fmaf(-u_l, load_constant(&Tu->spec.d, 0), load_constant(&Tu->spec.a, 0)));
         }
-        const float3 med = ppd::table_lerp_slopes(T.table, T.xmax, (lambda - T.lmin) * T.inv_dl);
+        const float3 med = ppd::table_lerp_slopes(
+            load_constant(&Tu->table, 0), load_constant(&Tu->xmax, 0),
+            (lambda - load_constant(&Tu->lmin, 0)) * load_constant(&Tu->inv_dl, 0));
         const float n_phase = med.x;
         const float sca_ln2 = med.y;  // the table holds λ_sca·ln2
         const float abs_len = med.z;
