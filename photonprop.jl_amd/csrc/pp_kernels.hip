@@ -378,33 +378,24 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
         const uint32_t w_lone = w0.y;  // the lone final step's length word (cone)
         PP_STEP(0u, ppd::draw_a(w0.z), ppd::draw_a(w0.w), ppd::draw_p(w0.z, w0.w), false,
                 max_steps == 1u)
-        for (uint32_t s0 = 1; s0 < max_steps; s0 += 4) {
+        // Pairs of steps s0, s0 + 1, one Philox block each; the first of a pair
+        // is never the final step.  A final step that opens a block (max_steps
+        // even) comes after the loop: it needs only its length, which under
+        // cone emission is the unused word of block 0 (no block computed).
+        uint32_t s0 = 1;
+        for (; s0 + 1 < max_steps; s0 += 2) {
             ppd::U4 Q;
-            if (alive) {
-                if (cone && s0 + 1 >= max_steps)
-                    Q = ppd::U4{w_lone, w_lone, w_lone, w_lone};
-                else
-                    Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo,
-                                      R.seed_hi);
-            }
-            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false,
-                    s0 + 1 >= max_steps)
-            if (s0 + 1 >= max_steps) break;
+            if (alive)
+                Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+            PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false, false)
             PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w),
                     (s0 & 7u) == 5u, s0 + 2 >= max_steps)
-            if (s0 + 2 >= max_steps) break;
-            if (alive) {
-                if (cone && s0 + 3 >= max_steps)
-                    Q = ppd::U4{w_lone, w_lone, w_lone, w_lone};
-                else
-                    Q = ppd::philox10((s0 + 3u) >> 1, id_lo, id_hi, R.stream, R.seed_lo,
-                                      R.seed_hi);
-            }
-            PP_STEP(s0 + 2u, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false,
-                    s0 + 3 >= max_steps)
-            if (s0 + 3 >= max_steps) break;
-            PP_STEP(s0 + 3u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w), false,
-                    s0 + 4 >= max_steps)
+        }
+        if (s0 < max_steps) {
+            uint32_t wl = w_lone;
+            if (!cone && alive)
+                wl = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi).x;
+            PP_STEP(s0, ppd::draw_a(wl), 1.0f, 1.0f, false, true)
         }
 #undef PP_STEP
         acc_steps += nst;
