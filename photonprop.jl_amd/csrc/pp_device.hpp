@@ -42,8 +42,16 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 // XOR, then at most one vector XOR); a v_bitop3_b32 would take only one
 // scalar operand and need a v_mov for the other.  The grouping is only
 // associativity, so any counter (uniform or not) gives the same words.
-__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1) {
+// With `rk` (the key's schedule in LDS, rk[r] = (k0 + r·W0, k1 + r·W1), see
+// philox_keys_to_lds), the keys that only feed vector XORs (k0 of rounds 3-9,
+// k1 of rounds 4-9) come from LDS reads instead of scalar adds: a round's key
+// add and its use had been a scalar-to-vector dependency in every round, and
+// the 13 adds per block took scalar issue slots the step loop is short of.
+// The keys of the first rounds, which combine with uniform words on the scalar
+// unit, stay scalar adds.  Without `rk` every key is a scalar add.
+template <bool kRk>
+__device__ __forceinline__ U4 philox10_t(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                         uint32_t k0, uint32_t k1, const uint2* rk) {
     // opaque key: each round's key is one scalar add from the last, instead of
     // a 20-register key schedule that the compiler would hoist out of the loops
     k0 = __builtin_amdgcn_readfirstlane(k0);
@@ -65,20 +73,41 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
             n0 = h1 ^ (c1 ^ k0);
             n2 = c3 ^ (h0 ^ k1);
         } else if (r == 3) {
-            n0 = xor3(h1, c1, k0);
+            n0 = xor3(h1, c1, kRk ? rk[3].x : k0);
             n2 = h0 ^ (c3 ^ k1);
         } else {
-            n0 = xor3(h1, c1, k0);
-            n2 = xor3(h0, c3, k1);
+            const uint2 kk = kRk ? rk[r] : make_uint2(k0, k1);
+            n0 = xor3(h1, c1, kk.x);
+            n2 = xor3(h0, c3, kk.y);
         }
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
         c2 = n2;
-        k0 += kW0;
-        k1 += kW1;
+        if (!kRk || r < 3) {
+            k0 += kW0;
+            k1 += kW1;
+        }
     }
     return U4{c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1) {
+    return philox10_t<false>(c0, c1, c2, c3, k0, k1, nullptr);
+}
+
+__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, const uint2* rk) {
+    return philox10_t<true>(c0, c1, c2, c3, k0, k1, rk);
+}
+
+// The schedule of key (k0, k1) for philox10's `rk`: lane r < 10 of a wave
+// writes round key r.  A wave reads its own entries only, and LDS operations
+// of one wave complete in order, so no barrier is needed before the reads.
+__device__ __forceinline__ void philox_keys_to_lds(uint2* rk, uint32_t lane, uint32_t k0,
+                                                   uint32_t k1) {
+    if (lane < 10u) rk[lane] = make_uint2(k0 + lane * kW0, k1 + lane * kW1);
 }
 
 // Uniform on (0, 1]: (top 24 bits + 1) * 2^-24, exact in FP32 (self-tests).

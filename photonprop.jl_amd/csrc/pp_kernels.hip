@@ -55,6 +55,7 @@ namespace {
 constexpr int kBlock = 256;                       // lanes per block (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kRoundsPerTile = 32;            // photons per lane per tile
+constexpr int kKeyRounds = 10;                     // Philox round keys per request (LDS)
 static_assert((uint64_t)kBlock * kRoundsPerTile == kTilePhotons, "a tile is kRoundsPerTile rounds of a block");
 constexpr double kCVacuum = 0.299792458;           // m/ns
 constexpr float kLn2 = 0.69314718055994530942f;
@@ -219,11 +220,15 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
                                                 unsigned long long* hit_cursor,
                                                 double (*s_red)[3], uint64_t (*s_redi)[2],
                                                 uint4 (*s_hitbuf)[kHitBuf * 4],
-                                                float (*s_idir)[kBlock]) {
+                                                float (*s_idir)[kBlock],
+                                                uint2 (*s_keys)[kKeyRounds]) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
     uint4* const hbuf = s_hitbuf[wave];
+    // the request's Philox key schedule, one copy per wave
+    const uint2* const rk = s_keys[wave];
+    ppd::philox_keys_to_lds(s_keys[wave], lane, R.seed_lo, R.seed_hi);
     uint32_t fill = 0;  // records staged in hbuf (wave-uniform)
     // The hits' count and FP64 sums accumulate per wave in s_red / s_redi
     // (lane 0 owns its wave's row; the caller's barrier before the tile has
@@ -262,7 +267,7 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
         // step of its block and the emission is the Cherenkov cone (which
         // leaves a(y) of block 0 unused), its length is a(y) of block 0 and
         // its block is never computed.
-        const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+        const ppd::U4 w0 = ppd::philox10(0u, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi, rk);
         const bool cone = R.profile == PP_EMIT_CHERENKOV_CONE;
         const float u_l = 2.0f - ppd::draw_a(w0.x);
         float lambda;
@@ -386,7 +391,8 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
         for (; s0 + 1 < max_steps; s0 += 2) {
             ppd::U4 Q;
             if (alive)
-                Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi);
+                Q = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi,
+                                  rk);
             PP_STEP(s0, ppd::draw_a(Q.x), ppd::draw_a(Q.y), ppd::draw_p(Q.x, Q.y), false, false)
             PP_STEP(s0 + 1u, ppd::draw_a(Q.z), ppd::draw_a(Q.w), ppd::draw_p(Q.z, Q.w),
                     (s0 & 7u) == 5u, s0 + 2 >= max_steps)
@@ -394,7 +400,8 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
         if (s0 < max_steps) {
             uint32_t wl = w_lone;
             if (!cone && alive)
-                wl = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi).x;
+                wl = ppd::philox10((s0 + 1u) >> 1, id_lo, id_hi, R.stream, R.seed_lo, R.seed_hi,
+                                   rk).x;
             PP_STEP(s0, ppd::draw_a(wl), 1.0f, 1.0f, false, true)
         }
 #undef PP_STEP
@@ -467,12 +474,13 @@ __device__ __forceinline__ TilePartial run_tile_any(const TileCtx* Tc, const Dev
                                                     unsigned long long* hit_cursor,
                                                     double (*s_red)[3], uint64_t (*s_redi)[2],
                                                     uint4 (*s_hitbuf)[kHitBuf * 4],
-                                                    float (*s_idir)[kBlock]) {
+                                                    float (*s_idir)[kBlock],
+                                                    uint2 (*s_keys)[kKeyRounds]) {
     if (load_constant(&Tc->hg, 0).iso)
         return run_tile<kSysStores, true>(Tc, R, req_index, first, count, hits, hits_cap,
-                                          hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
+                                          hit_cursor, s_red, s_redi, s_hitbuf, s_idir, s_keys);
     return run_tile<kSysStores, false>(Tc, R, req_index, first, count, hits, hits_cap,
-                                       hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
+                                       hit_cursor, s_red, s_redi, s_hitbuf, s_idir, s_keys);
 }
 
 
@@ -493,6 +501,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     __shared__ float s_idir[3][kBlock];
+    __shared__ uint2 s_keys[kWaves][kKeyRounds];
     for (;;) {
         __syncthreads();  // previous tile's s_tile and s_red consumed
         if (threadIdx.x == 0) s_tile = atomicAdd(P.tile_counter, 1u);
@@ -514,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t left = R.n_photons - first;
         const uint32_t count = (uint32_t)(left < kTilePhotons ? left : kTilePhotons);
         const TilePartial tp = run_tile_any<false>(kernarg_tile(), R, r, first, count, P.hits, P.hits_cap,
-                                                   P.hit_cursor, s_red, s_redi, s_hitbuf, s_idir);
+                                                   P.hit_cursor, s_red, s_redi, s_hitbuf, s_idir, s_keys);
         if (threadIdx.x == 0) P.partials[tile] = tp;
     }
 }
@@ -1619,6 +1628,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ uint64_t s_redi[kWaves][2];
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     __shared__ float s_idir[3][kBlock];
+    __shared__ uint2 s_keys[kWaves][kKeyRounds];
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_desc_slot = ~0u;
 
@@ -1752,7 +1762,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         TilePartial* parts = SA(partials) + (uint64_t)slot * SA(max_tiles);
         const TilePartial tp = run_tile_any<true>(&opaque_ptr(P.addr)->tile, R, slot, first, count, SA(pool) + hit_base,
                                                   hit_cap, &SA(hit_cursor)[slot], s_red, s_redi,
-                                                  s_hitbuf, s_idir);
+                                                  s_hitbuf, s_idir, s_keys);
         // every wave waits for its own (system-scope) hit stores to complete
         // before the barrier; a workgroup-scope fence would not wait for them
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
