@@ -199,7 +199,9 @@ int pp_propagate(pp_ctx* ctx, const pp_emitter* emitter, const pp_spectrum* spec
  * request r owns hits_out[offsets_out[r] .. offsets_out[r+1]).
  * hits_cap bounds all hits of the launch (exceeding it is PP_ERR_OVERFLOW);
  * per_request_cap (0: none) keeps at most that many records of each request,
- * a subset independent of the photons' histories.
+ * a subset independent of the photons' histories.  Within a request, records
+ * are in the order the kernel's waves flushed them, which varies from run to
+ * run; so does the subset a cap keeps (the summaries do not vary).
  * offsets_out: n_req+1 entries; summaries_out: n_req entries. */
 int pp_propagate_batch(pp_ctx* ctx, const pp_request* requests, uint32_t n_req,
                        const pp_spectrum* spectrum, pp_hit* hits_out, uint64_t hits_cap,

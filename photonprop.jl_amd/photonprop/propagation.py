@@ -169,7 +169,9 @@ class Context:
         """Propagate many requests in one persistent launch (pp_propagate_batch).
         `requests` is a REQUEST_DTYPE array.  Returns (hits, offsets, summaries):
         request r owns hits[offsets[r]:offsets[r+1]].  per_request_cap > 0 keeps
-        at most that many records per request."""
+        at most that many records per request.  A request's records come in the
+        order the kernel's waves flushed them, which varies from run to run, and
+        so does the subset a cap keeps; the summaries are the same on every run."""
         reqs = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
         n = reqs.shape[0]
         cap = int(min(int(reqs["n_photons"].sum()), DEFAULT_HITS_CAP)

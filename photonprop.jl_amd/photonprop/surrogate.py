@@ -126,7 +126,8 @@ def generate_training_set(ctx, n_rows, photons_per_row=200_000, rows_per_medium=
     """MC rows from the batched kernel: each group of rows_per_medium rows shares
     a randomly drawn medium; each row is one request of photons_per_row photons
     with up to n_times hit times kept (a subset independent of the photons'
-    histories).  `allocate` (optional) maps a group's raw features to each
+    histories, in the kernel's flush order, so it can differ between runs while
+    the rows' counts and weights do not).  `allocate` (optional) maps a group's raw features to each
     row's photon count instead (see YieldAllocator); the Poisson likelihood
     takes every row's own exposure, so rows may differ."""
     spectrum = spectrum or InverseSquareSpectrum(300.0, 600.0)
