@@ -107,6 +107,7 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
 // of one wave complete in order, so no barrier is needed before the reads.
 __device__ __forceinline__ void philox_keys_to_lds(uint2* rk, uint32_t lane, uint32_t k0,
                                                    uint32_t k1) {
+    __asm__ volatile("" : "+v"(lane));
     if (lane < 10u) rk[lane] = make_uint2(k0 + lane * kW0, k1 + lane * kW1);
 }
 

@@ -179,6 +179,7 @@ template <bool kSysStores>
 __device__ __forceinline__ void flush_hits(const uint4* buf, uint32_t n, pp_hit* hits,
                                            uint64_t hits_cap, unsigned long long* cursor,
                                            uint32_t lane, double* wsum, uint64_t* whits) {
+    __asm__ volatile("" : "+v"(lane));
     double w = 0.0, wt = 0.0, wt2 = 0.0;
     if (lane < n) {
         const double t = (double)__uint_as_float(buf[4 * lane].w);
@@ -230,14 +231,24 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
     const uint2* const rk = s_keys[wave];
     ppd::philox_keys_to_lds(s_keys[wave], lane, R.seed_lo, R.seed_hi);
     uint32_t fill = 0;  // records staged in hbuf (wave-uniform)
+    // The wave's row of s_red / s_redi is addressed through a wave index taken
+    // afresh at each use (an empty asm stops the compiler from computing the
+    // row addresses once and holding them, spilled, through the step loop;
+    // the uses are per flush and per tile).
+    auto row_wave = [tid]() {
+        uint32_t w = tid >> 6;
+        __asm__ volatile("" : "+v"(w));
+        return w;
+    };
     // The hits' count and FP64 sums accumulate per wave in s_red / s_redi
     // (lane 0 owns its wave's row; the caller's barrier before the tile has
     // released the previous tile's rows), added at each flush.
     if (lane == 0) {
-        s_red[wave][0] = 0.0;
-        s_red[wave][1] = 0.0;
-        s_red[wave][2] = 0.0;
-        s_redi[wave][1] = 0ull;
+        const uint32_t rw = row_wave();
+        s_red[rw][0] = 0.0;
+        s_red[rw][1] = 0.0;
+        s_red[rw][2] = 0.0;
+        s_redi[rw][1] = 0ull;
     }
     const uint32_t rounds = (count + kBlock - 1) / kBlock;
     const uint32_t max_steps = R.max_steps;
@@ -412,8 +423,9 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
         if (mask != 0) {
             const uint32_t h = (uint32_t)__popcll(mask);
             if (fill + h > kHitBuf) {
-                flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane, s_red[wave],
-                                       &s_redi[wave][1]);
+                const uint32_t rw = row_wave();
+                flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane, s_red[rw],
+                                       &s_redi[rw][1]);
                 fill = 0;
             }
             if (hit) {
@@ -440,13 +452,15 @@ __device__ __forceinline__ TilePartial run_tile(const TileCtx* Tc, const DevReq&
             fill += h;
         }
     }
-    if (fill)
-        flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane, s_red[wave],
-                               &s_redi[wave][1]);
+    if (fill) {
+        const uint32_t rw = row_wave();
+        flush_hits<kSysStores>(hbuf, fill, hits, hits_cap, hit_cursor, lane, s_red[rw],
+                               &s_redi[rw][1]);
+    }
 
     // ---- tile partial: fixed-order reduction ----
     const unsigned long long ws = wave_sum_u64(acc_steps);
-    if (lane == 0) s_redi[wave][0] = ws;
+    if (lane == 0) s_redi[row_wave()][0] = ws;
     __syncthreads();
     TilePartial tp;
     tp.n_steps = 0;
