@@ -1643,10 +1643,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ uint4 s_hitbuf[kWaves][kHitBuf * 4];
     __shared__ float s_idir[3][kBlock];
     __shared__ uint2 s_keys[kWaves][kKeyRounds];
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_desc_slot = ~0u;
+    if (threadIdx.x == 0) s_desc_slot = ~0u;
 
     for (;;) {
+        // the thread index taken afresh per tile (an empty asm): the per-lane
+        // addresses of the scheduling code are recomputed per tile instead of
+        // being held, spilled, through the whole kernel
+        uint32_t tid = threadIdx.x;
+        __asm__ volatile("" : "+v"(tid));
         __syncthreads();  // the previous tile's shared words consumed
         if (tid < 64) {   // wave 0: ticket, gate, range entry, slot descriptor
             const uint32_t lane = tid;
