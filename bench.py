@@ -33,7 +33,7 @@ HIT_BYTES = 64
 # 1 GiB device copy (profiles/propagate_kernel_64geom_1e9.md).  Reported only
 # for that workload.
 PROFILED_PHOTONS = 1e9
-PROFILED_TRAFFIC_BYTES = 1.92143e10
+PROFILED_TRAFFIC_BYTES = 1.77844e10
 
 
 def workload_requests(pp, photons_per_geometry, seed=0):
