@@ -158,6 +158,9 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
 // Measured and dropped: 64x64 tiles per wave (each LDS fragment feeding two
 // MFMAs, each weight fragment read by two waves) ran at half the speed, as
 // the L2 requests doubled; the L1 does not merge the second wave's reads.
+// 128x64 tiles per wave (4 waves per block, each weight fragment still read
+// once per block, half the LDS reads) ran 6-11 % slower: 255 VGPRs leave 2
+// waves per SIMD, too few to hide the weights' L2 latency.
 constexpr uint32_t kFusedRows = 128;      // 4 sample tiles of 32
 constexpr uint32_t kFusedMaxWidth = 256;  // 8 feature tiles: one per wave
 constexpr uint32_t kFusedLd = kFusedMaxWidth + 8;  // LDS row stride (bf16): rows 16 B apart
