@@ -139,10 +139,10 @@ def summarize(res, n_req):
     }
 
 
-MIXED_RATES = (500.0, 1000.0, 1500.0, 2000.0, 2400.0, 2800.0, 3000.0, 3200.0, 3400.0, 5000.0,
-               5500.0, 6000.0, 6500.0, 20000.0)
-SMALL_RATES = (2000.0, 5000.0, 10000.0, 20000.0, 40000.0, 60000.0, 80000.0, 120000.0, 160000.0,
-               240000.0, 400000.0, 800000.0)
+MIXED_RATES = (500.0, 1000.0, 1500.0, 2000.0, 2400.0, 2800.0, 3000.0, 3200.0, 3400.0, 3600.0,
+               3800.0, 5000.0, 5500.0, 6000.0, 6500.0, 7000.0, 20000.0)
+SMALL_RATES = (2000.0, 5000.0, 7000.0, 8000.0, 9000.0, 10000.0, 20000.0, 40000.0, 60000.0,
+               80000.0, 120000.0, 160000.0, 240000.0, 400000.0, 500000.0, 800000.0)
 
 
 def _serving_sweep(pp, ctx, spectrum, n_req, rates, photons, label):
