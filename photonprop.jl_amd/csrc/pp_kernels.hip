@@ -547,7 +547,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
 template <bool kSysLoads>
 __device__ __forceinline__ pp_summary reduce_partials(const TilePartial* parts, uint64_t n_tiles,
                                                       uint64_t n_sim) {
-    const uint32_t lane = threadIdx.x & 63u;
+    // (an empty asm: the per-lane offsets are formed here, not hoisted into a
+    // persistent caller's registers)
+    uint32_t lane = threadIdx.x & 63u;
+    __asm__ volatile("" : "+v"(lane));
     uint64_t st = 0, nh = 0;
     double sw = 0.0, swt = 0.0, swt2 = 0.0;
     for (uint64_t t = lane; t < n_tiles; t += 64) {
@@ -1572,6 +1575,15 @@ __device__ __forceinline__ void agent_store(T* p, T v) {
 }
 
 // A value that is the same in every lane, moved to scalar registers.
+// kGateStop as an atomic's operand, formed in scalar registers where it is
+// used (an empty asm): hoisted, the 64-bit constant was held in two VGPRs
+// through the whole kernel and spilled
+__device__ __forceinline__ unsigned long long stop_bit() {
+    unsigned long long v = kGateStop;
+    __asm__ volatile("" : "+s"(v));
+    return v;
+}
+
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
@@ -1656,7 +1668,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
             const uint32_t lane = tid;
             unsigned long long tk = 0;
             if (lane == 0) tk = atomicAdd(SA(ticket), 1ull);
-            const uint64_t t = uniform_u64(__shfl(tk, 0, 64));
+            // lane 0 took the ticket; it is the first active lane (the
+            // whole wave is here), so a first-lane read broadcasts it
+            const uint64_t t = uniform_u64(tk);
             uint64_t got = kSrvExit;
             uint64_t wait_start = 0;  // device clock when this block began to wait as head
             for (;;) {
@@ -1679,7 +1693,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
                         break;
                     }
                     if (uniform_u64(sys_load(&SA(ctl)->stop))) {
-                        if (lane == 0) atomicOr(SA(gate), kGateStop);
+                        if (lane == 0) atomicOr(SA(gate), stop_bit());
                         break;
                     }
                     // nothing published for wait_ticks: stop, and report the
@@ -1692,7 +1706,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
                         if (lane == 0) {
                             __hip_atomic_store(&SA(ctl)->self_stop, (uint64_t)(t + 1),
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                            atomicOr(SA(gate), kGateStop);
+                            atomicOr(SA(gate), stop_bit());
                         }
                         break;
                     }

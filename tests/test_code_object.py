@@ -85,8 +85,8 @@ def test_propagate_kernel_fits_eight_waves_without_spills(kernels):
 def test_serve_kernel_keeps_the_batch_kernels_occupancy(kernels):
     for name, md in _find(kernels, "serve_kernel").items():
         assert md["vgpr_count"] <= 64, (name, md)
-        # a few per-lane values of the tile-claiming code, reloaded per claim
-        assert md["private_segment_fixed_size"] <= 32, (name, md)
+        assert md["vgpr_spill_count"] == 0, (name, md)
+        assert md["private_segment_fixed_size"] == 0, (name, md)
 
 
 def test_fused_mlp_forward_without_spills(kernels):
