@@ -150,6 +150,15 @@ CASES = [
     ("uniform_sphere", dict(n=300_000, profile="UniformSphere"), (-2.0, 1.0, 0.0), 0.5, 10, 2, 0),
     ("ref_isotropic", dict(n=300_000, profile="IsotropicEmission"), (0.0, 0.0, 3.0), 0.4, 10, 0, 0),
     ("one_step", dict(n=300_000), (1.0, 0.0, 0.9), 0.4, 1, 4, 0),
+    # the step loop's edges: the lone final step from block 0's spare word
+    # (cone, even), one pair ending on the final step (cone, odd), and a final
+    # pair / lone step that need their own Philox block (not the cone)
+    ("two_steps_cone", dict(n=300_000), (1.0, 0.2, 0.9), 0.4, 2, 6, 0),
+    ("three_steps_cone", dict(n=300_000), (0.9, -0.3, 1.0), 0.4, 3, 7, 2),
+    ("five_steps_isotropic", dict(n=300_000, profile="IsotropicEmission"), (0.0, 1.0, 1.5), 0.4, 5,
+     10, 0),
+    ("four_steps_isotropic", dict(n=300_000, profile="IsotropicEmission"), (1.0, 1.0, 0.0), 0.4, 4,
+     11, 1),
     ("ragged_tile", dict(n=8192 * 3 + 77), (1.2, 0.0, 1.1), 0.5, 10, 8, 0),
     ("tiny", dict(n=5), (0.5, 0.0, 0.45), 0.3, 10, 0, 0),
 ]
